@@ -1,0 +1,1073 @@
+// CPU ORACLE — test infrastructure only.
+//
+// FP64 CPU restatement of Atum.jl's DGSEM tendency and time-step path.  It is
+// the checker for the HIP library: only the tests, the smoke check and the
+// bench's CPU-baseline leg load it.  The product library never links it.
+//
+// Restated from (paths relative to the reference checkout):
+//   src/dgsem.jl:38-56      DGSEM constructor (MJ, MJI, faceMJ)
+//   src/dgsem.jl:59-98      RHS functor: volume then surface
+//   src/dgsem.jl:217-306    volumeterm!          (weak form)
+//   src/dgsem.jl:308-468    esvolumeterm!        (flux differencing, :naive)
+//   src/dgsem.jl:470-561    esvolumeterm_dir!    (flux differencing, :per_dir)
+//   src/dgsem.jl:563-681    esvolumeterm_dir_symmetric! (:per_dir_symmetric)
+//   src/dgsem.jl:683-732    surfaceterm!
+//   src/odesolvers.jl:9-229 solve!, LSRK, RLSRK, coefficient tables
+//   src/odesolvers.jl:231-388 ARK / ARK23
+//   src/bandedsystem.jl:1-15 banded matrix by probing the linear operator
+//   src/measures.jl:6-33    weightednorm, entropyintegral, entropyproduct
+// Bennu (external, un-vendored, compat 0.3) pieces are reconstructed from their
+// call sites: LobattoCell operators, brickgrid (Cartesian and stacked
+// ordering), affine metrics, face tables, banded LU and solve.
+#include <omp.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <vector>
+
+#include "laws.hpp"
+
+using namespace orc;
+
+extern "C" {
+
+struct orc_geom {
+  int32_t dim, Nq, K, naux;
+  const double* D;       // Nq*Nq, column major: D[r + Nq*c] = l_c'(xi_r)
+  const double* g;       // dim*dim components (a + dim*b) = d xi_a / d x_b, each Np*K
+  const double* MJ;      // Np*K
+  const double* MJI;     // Np*K
+  const double* faceMJ;  // Nfp*K, Nfp = 2*dim*Nq^(dim-1)
+  const double* normal;  // dim components, each Nfp*K
+  const double* aux;     // naux components, each Np*K
+  const int64_t* faceix_m;  // Nfp*K, 0-based linear node index
+  const int64_t* faceix_p;  // Nfp*K
+  const int32_t* bctag;     // 2*dim*K
+};
+
+struct orc_dg {
+  orc_geom geom;
+  int32_t law;
+  int32_t volume_form;  // 0 weak, 1 flux differencing
+  int32_t fd_kernel;    // 0 per_dir, 1 per_dir_symmetric, 2 naive
+  int32_t vflux, sflux;
+  int32_t dirmask;      // bit r set <=> direction r+1 in `directions`
+  int32_t bc, source;
+  int32_t pde_level_balance;
+  int32_t n_bc_states;
+  double gamma, grav, omega;
+  double u[3];
+  const double* bc_states;
+};
+
+}  // extern "C"
+
+namespace {
+
+struct Ctx {
+  const orc_dg* o;
+  Params P;
+  int dim, Nq, Np, K, naux, Nfpf, Nfp;
+  int64_t Ntot;
+  int dirs[3];
+  int ndir;
+  double Dm(int r, int c) const { return o->geom.D[r + Nq * c]; }
+  double gm(int a, int b, int64_t node) const { return o->geom.g[(int64_t)(a + dim * b) * Ntot + node]; }
+};
+
+Ctx make_ctx(const orc_dg* o) {
+  Ctx c;
+  c.o = o;
+  c.P.gamma = o->gamma;
+  c.P.grav = o->grav;
+  for (int d = 0; d < 3; ++d) c.P.u[d] = o->u[d];
+  c.P.pde_level_balance = o->pde_level_balance;
+  c.P.bc = o->bc;
+  c.P.source = o->source;
+  c.P.omega = o->omega;
+  c.P.n_bc_states = o->n_bc_states;
+  c.P.bc_states = o->bc_states;
+  c.dim = o->geom.dim;
+  c.Nq = o->geom.Nq;
+  c.Np = 1;
+  for (int d = 0; d < c.dim; ++d) c.Np *= c.Nq;
+  c.K = o->geom.K;
+  c.naux = o->geom.naux;
+  c.Nfpf = c.Np / c.Nq;
+  c.Nfp = 2 * c.dim * c.Nfpf;
+  c.Ntot = (int64_t)c.Np * c.K;
+  c.ndir = 0;
+  for (int r = 0; r < c.dim; ++r)
+    if (o->dirmask & (1 << r)) c.dirs[c.ndir++] = r;
+  return c;
+}
+
+template <int Ns>
+inline void gather(const double* f, int64_t Ntot, int64_t node, double* v) {
+  for (int s = 0; s < Ns; ++s) v[s] = f[s * Ntot + node];
+}
+inline void gather_aux(const Ctx& c, int64_t node, double* a) {
+  for (int s = 0; s < c.naux; ++s) a[s] = c.o->geom.aux[s * c.Ntot + node];
+}
+
+constexpr int MAXAUX = 16;
+
+// src/dgsem.jl:217-306
+template <class L>
+void volume_weak(const Ctx& c, const double* q, double* dq, bool increment) {
+  constexpr int D = L::D, Ns = L::Ns;
+  const int Nq = c.Nq, Np = c.Np, ndir = c.ndir;
+  const int Nq2 = D > 1 ? Nq : 1;
+#pragma omp parallel
+  {
+    std::vector<double> lF((size_t)Np * ndir * Ns);
+#pragma omp for schedule(static)
+    for (int e = 0; e < c.K; ++e) {
+      for (int ijk = 0; ijk < Np; ++ijk) {
+        int64_t node = ijk + (int64_t)Np * e;
+        double qn[Ns], an[MAXAUX], F[D][Ns];
+        gather<Ns>(q, c.Ntot, node, qn);
+        gather_aux(c, node, an);
+        L::flux(c.P, qn, an, F);
+        double MJn = c.o->geom.MJ[node];
+        for (int s = 0; s < Ns; ++s)
+          for (int dl = 0; dl < ndir; ++dl) {
+            int dir = c.dirs[dl];
+            double v = 0;
+            for (int dd = 0; dd < D; ++dd) v += c.gm(dir, dd, node) * F[dd][s];
+            v *= MJn;
+            lF[((size_t)ijk * ndir + dl) * Ns + s] = v;
+          }
+      }
+      for (int ijk = 0; ijk < Np; ++ijk) {
+        int i = ijk % Nq, j = (ijk / Nq) % Nq2, k = ijk / (Nq * Nq2);
+        int64_t node = ijk + (int64_t)Np * e;
+        double qn[Ns], an[MAXAUX], dql[Ns];
+        gather<Ns>(q, c.Ntot, node, qn);
+        gather_aux(c, node, an);
+        for (int s = 0; s < Ns; ++s) dql[s] = -0.0;
+        L::law_source(c.P, dql, qn, an, c.o->dirmask);
+        L::problem_source(c.P, dql, qn, an, c.o->dirmask);
+        L::nonconservative(c.P, dql, qn, an, c.o->dirmask);
+        double MJI = c.o->geom.MJI[node];
+        for (int n = 0; n < Nq; ++n) {
+          double Dni = c.Dm(n, i) * MJI;
+          double Dnj = c.Dm(n, j) * MJI;
+          double Dnk = c.Dm(n, k) * MJI;
+          for (int s = 0; s < Ns; ++s) {
+            int dl = 0;
+            if (c.o->dirmask & 1) {
+              int id = n + Nq * (j + Nq2 * k);
+              dql[s] += Dni * lF[((size_t)id * ndir + dl) * Ns + s];
+              dl++;
+            }
+            if (c.o->dirmask & 2) {
+              int id = i + Nq * (n + Nq2 * k);
+              dql[s] += Dnj * lF[((size_t)id * ndir + dl) * Ns + s];
+              dl++;
+            }
+            if (c.o->dirmask & 4) {
+              int id = i + Nq * (j + Nq2 * n);
+              dql[s] += Dnk * lF[((size_t)id * ndir + dl) * Ns + s];
+            }
+          }
+        }
+        for (int s = 0; s < Ns; ++s) {
+          double* p = dq + s * c.Ntot + node;
+          *p = increment ? *p + dql[s] : dql[s];
+        }
+      }
+    }
+  }
+}
+
+inline int line_node(int dir, int n, int i, int j, int k, int Nq, int Nq2) {
+  if (dir == 0) return n + Nq * (j + Nq2 * k);
+  if (dir == 1) return i + Nq * (n + Nq2 * k);
+  return i + Nq * (j + Nq2 * n);
+}
+
+// src/dgsem.jl:470-561 (:per_dir) and :563-681 (:per_dir_symmetric)
+template <class L>
+void volume_fd_dir(const Ctx& c, const double* q, double* dq, bool increment, int dir, bool symmetric) {
+  constexpr int D = L::D, Ns = L::Ns;
+  const int Nq = c.Nq, Np = c.Np;
+  const int Nq2 = D > 1 ? Nq : 1;
+  const int vf = c.o->vflux;
+#pragma omp parallel
+  {
+    std::vector<double> lg((size_t)Np * 3);
+#pragma omp for schedule(static)
+    for (int e = 0; e < c.K; ++e) {
+      for (int ijk = 0; ijk < Np; ++ijk) {
+        int64_t node = ijk + (int64_t)Np * e;
+        double MJn = c.o->geom.MJ[node];
+        for (int d = 0; d < D; ++d) lg[ijk * 3 + d] = MJn * c.gm(dir, d, node);
+      }
+      for (int ijk = 0; ijk < Np; ++ijk) {
+        int i = ijk % Nq, j = (ijk / Nq) % Nq2, k = ijk / (Nq * Nq2);
+        int idx[3] = {i, j, k};
+        int id = idx[dir];
+        int64_t node = ijk + (int64_t)Np * e;
+        double q1[Ns], a1[MAXAUX], d1[Ns], d2[Ns];
+        gather<Ns>(q, c.Ntot, node, q1);
+        gather_aux(c, node, a1);
+        for (int s = 0; s < Ns; ++s) { d1[s] = -0.0; d2[s] = -0.0; }
+        L::law_source(c.P, d1, q1, a1, 1 << dir);
+        L::problem_source(c.P, d1, q1, a1, 1 << dir);
+        double MJI = c.o->geom.MJI[node];
+        for (int pass = 0; pass < (symmetric ? 2 : 1); ++pass) {
+          double* acc = pass == 0 ? d1 : d2;
+          for (int nn = 0; nn < Nq; ++nn) {
+            int n = pass == 0 ? nn : Nq - 1 - nn;
+            int ild = line_node(dir, n, i, j, k, Nq, Nq2);
+            int64_t nnode = ild + (int64_t)Np * e;
+            double q2[Ns], a2[MAXAUX], F[D][Ns];
+            gather<Ns>(q, c.Ntot, nnode, q2);
+            gather_aux(c, nnode, a2);
+            L::twopoint(vf, c.P, q1, a1, q2, a2, F);
+            for (int s = 0; s < Ns; ++s) {
+              double Ddn = MJI * c.Dm(id, n);
+              double Dnd = MJI * c.Dm(n, id);
+              for (int d = 0; d < D; ++d) {
+                if (symmetric) {
+                  acc[s] -= Ddn * lg[ijk * 3 + d] * F[d][s] / 2;
+                  acc[s] += F[d][s] * lg[ild * 3 + d] * Dnd / 2;
+                } else {
+                  acc[s] -= Ddn * lg[ijk * 3 + d] * F[d][s];
+                  acc[s] += F[d][s] * lg[ild * 3 + d] * Dnd;
+                }
+              }
+            }
+          }
+        }
+        for (int s = 0; s < Ns; ++s) {
+          double v = symmetric ? d1[s] + d2[s] : d1[s];
+          double* p = dq + s * c.Ntot + node;
+          *p = increment ? *p + v : v;
+        }
+      }
+    }
+  }
+}
+
+// src/dgsem.jl:308-468 (:naive, all directions in one pass; requires all directions)
+template <class L>
+void volume_fd_naive(const Ctx& c, const double* q, double* dq, bool increment) {
+  constexpr int D = L::D, Ns = L::Ns;
+  const int Nq = c.Nq, Np = c.Np;
+  const int Nq2 = D > 1 ? Nq : 1;
+  const int vf = c.o->vflux;
+#pragma omp parallel for schedule(static)
+  for (int e = 0; e < c.K; ++e) {
+    for (int ijk = 0; ijk < Np; ++ijk) {
+      int i = ijk % Nq, j = (ijk / Nq) % Nq2, k = ijk / (Nq * Nq2);
+      int64_t node = ijk + (int64_t)Np * e;
+      double q1[Ns], a1[MAXAUX], dql[Ns];
+      gather<Ns>(q, c.Ntot, node, q1);
+      gather_aux(c, node, a1);
+      for (int s = 0; s < Ns; ++s) dql[s] = -0.0;
+      L::law_source(c.P, dql, q1, a1, c.o->dirmask);
+      L::problem_source(c.P, dql, q1, a1, c.o->dirmask);
+      double MJI = 1 / c.o->geom.MJ[node];
+      int idx[3] = {i, j, k};
+      for (int n = 0; n < Nq; ++n) {
+        for (int dir = 0; dir < D; ++dir) {
+          int id = idx[dir];
+          int ild = line_node(dir, n, i, j, k, Nq, Nq2);
+          int64_t nnode = ild + (int64_t)Np * e;
+          double q2[Ns], a2[MAXAUX], F[D][Ns];
+          gather<Ns>(q, c.Ntot, nnode, q2);
+          gather_aux(c, nnode, a2);
+          L::twopoint(vf, c.P, q1, a1, q2, a2, F);
+          double MJ1 = c.o->geom.MJ[node], MJ2 = c.o->geom.MJ[nnode];
+          for (int s = 0; s < Ns; ++s) {
+            double Din = MJI * c.Dm(id, n);
+            double Dni = MJI * c.Dm(n, id);
+            for (int d = 0; d < D; ++d) {
+              double lg1 = (dir < 2) ? MJ1 * c.gm(dir, d, node) : c.gm(dir, d, node) * MJ1;
+              double lg2 = (dir < 2) ? MJ2 * c.gm(dir, d, nnode) : c.gm(dir, d, nnode) * MJ2;
+              dql[s] -= Din * lg1 * F[d][s];
+              dql[s] += F[d][s] * lg2 * Dni;
+            }
+          }
+        }
+      }
+      for (int s = 0; s < Ns; ++s) {
+        double* p = dq + s * c.Ntot + node;
+        *p = increment ? *p + dql[s] : dql[s];
+      }
+    }
+  }
+}
+
+// src/dgsem.jl:683-732
+template <class L>
+void surface(const Ctx& c, const double* q, double* dq) {
+  constexpr int D = L::D, Ns = L::Ns;
+  const int sf = c.o->sflux;
+  const int Nfpf = c.Nfpf, Nfp = c.Nfp;
+#pragma omp parallel for schedule(static)
+  for (int e = 0; e < c.K; ++e) {
+    for (int dl = 0; dl < c.ndir; ++dl) {
+      int dir = c.dirs[dl];
+      for (int side = 0; side < 2; ++side) {
+        int face = 2 * dir + side;
+        for (int i = 0; i < Nfpf; ++i) {
+          int64_t jj = face * Nfpf + i + (int64_t)Nfp * e;
+          int64_t idm = c.o->geom.faceix_m[jj];
+          double n[D];
+          for (int d = 0; d < D; ++d) n[d] = c.o->geom.normal[d * (int64_t)Nfp * c.K + jj];
+          double fMJ = c.o->geom.faceMJ[jj];
+          double qm[Ns], am[MAXAUX], qp[Ns], ap[MAXAUX], nf[Ns];
+          gather_aux(c, idm, am);
+          gather<Ns>(q, c.Ntot, idm, qm);
+          int tag = c.o->geom.bctag[face + 2 * D * (int64_t)e];
+          if (tag == 0) {
+            int64_t idp = c.o->geom.faceix_p[jj];
+            gather<Ns>(q, c.Ntot, idp, qp);
+            gather_aux(c, idp, ap);
+          } else {
+            boundarystate<L>(c.P, n, qm, tag, qp);
+            for (int s = 0; s < c.naux; ++s) ap[s] = am[s];
+          }
+          surfaceflux<L>(sf, c.P, n, qm, am, qp, ap, nf);
+          double MJI = c.o->geom.MJI[idm];
+          for (int s = 0; s < Ns; ++s) dq[s * c.Ntot + idm] -= fMJ * nf[s] * MJI;
+        }
+      }
+    }
+  }
+}
+
+template <class L>
+void tendency_t(const Ctx& c, const double* q, double* dq, bool increment) {
+  if (c.o->volume_form == 0) {
+    volume_weak<L>(c, q, dq, increment);
+  } else if (c.o->fd_kernel == 2) {
+    volume_fd_naive<L>(c, q, dq, increment);
+  } else {
+    for (int dl = 0; dl < c.ndir; ++dl)
+      volume_fd_dir<L>(c, q, dq, dl == 0 ? increment : true, c.dirs[dl], c.o->fd_kernel == 1);
+  }
+  surface<L>(c, q, dq);
+}
+
+// Probe that a law x flux combination exists in the reference.
+template <class L>
+bool supported_t(const Ctx& c) {
+  constexpr int D = L::D, Ns = L::Ns;
+  double q[Ns], a[MAXAUX], F[D][Ns], nf[Ns], n[D];
+  for (int s = 0; s < Ns; ++s) q[s] = 0.0;
+  q[0] = 1.0;
+  q[Ns - 1] = 3.0;
+  for (int s = 0; s < MAXAUX; ++s) a[s] = 1.0;
+  for (int d = 0; d < D; ++d) n[d] = d == 0 ? 1.0 : 0.0;
+  try {
+    if (c.o->volume_form == 1) L::twopoint(c.o->vflux, c.P, q, a, q, a, F);
+    surfaceflux<L>(c.o->sflux, c.P, n, q, a, q, a, nf);
+  } catch (Unsupported&) {
+    return false;
+  }
+  return true;
+}
+
+template <class L>
+double entropyintegral_t(const Ctx& c, const double* q) {
+  constexpr int Ns = L::Ns;
+  long double sum = 0;
+  for (int64_t node = 0; node < c.Ntot; ++node) {
+    double qn[Ns], an[MAXAUX];
+    gather<Ns>(q, c.Ntot, node, qn);
+    gather_aux(c, node, an);
+    sum += c.o->geom.MJ[node] * L::entropy(c.P, qn, an);
+  }
+  return (double)sum;
+}
+
+template <class L>
+double entropyproduct_t(const Ctx& c, const double* p, const double* q) {
+  constexpr int Ns = L::Ns;
+  long double sum = 0;
+  for (int64_t node = 0; node < c.Ntot; ++node) {
+    double pn[Ns], an[MAXAUX], v[Ns], qn[Ns];
+    gather<Ns>(p, c.Ntot, node, pn);
+    gather<Ns>(q, c.Ntot, node, qn);
+    gather_aux(c, node, an);
+    L::entropyvariables(c.P, pn, an, v);
+    double vq = v[0] * qn[0];
+    for (int s = 1; s < Ns; ++s) vq += v[s] * qn[s];
+    sum += c.o->geom.MJ[node] * vq;
+  }
+  return (double)sum;
+}
+
+// Dispatch on (law, dim) to the template instantiation.
+template <template <class> class Op, class... Args>
+auto dispatch(const Ctx& c, Args... args) {
+  int law = c.o->law, dim = c.dim;
+#define ORC_CASE(LAWID, TMPL, DIM) \
+  if (law == LAWID && dim == DIM) return Op<TMPL<DIM>>::run(c, args...);
+  ORC_CASE(LAW_ADVECTION, Advection, 1)
+  ORC_CASE(LAW_ADVECTION, Advection, 2)
+  ORC_CASE(LAW_ADVECTION, Advection, 3)
+  ORC_CASE(LAW_EULER, Euler, 1)
+  ORC_CASE(LAW_EULER, Euler, 2)
+  ORC_CASE(LAW_EULER, Euler, 3)
+  ORC_CASE(LAW_EULER_GRAVITY, EulerGravity, 1)
+  ORC_CASE(LAW_EULER_GRAVITY, EulerGravity, 2)
+  ORC_CASE(LAW_EULER_GRAVITY, EulerGravity, 3)
+  ORC_CASE(LAW_LINEAR_EULER_GRAVITY, LinearEulerGravity, 1)
+  ORC_CASE(LAW_LINEAR_EULER_GRAVITY, LinearEulerGravity, 2)
+  ORC_CASE(LAW_LINEAR_EULER_GRAVITY, LinearEulerGravity, 3)
+  ORC_CASE(LAW_SHALLOW_WATER, ShallowWater, 1)
+  ORC_CASE(LAW_SHALLOW_WATER, ShallowWater, 2)
+  ORC_CASE(LAW_SHALLOW_WATER, ShallowWater, 3)
+#undef ORC_CASE
+  return Op<Advection<1>>::fail();
+}
+
+template <class L>
+struct OpTendency {
+  static int run(const Ctx& c, const double* q, double* dq, bool inc) {
+    tendency_t<L>(c, q, dq, inc);
+    return 0;
+  }
+  static int fail() { return -1; }
+};
+template <class L>
+struct OpSupported {
+  static int run(const Ctx& c) { return supported_t<L>(c) ? 1 : 0; }
+  static int fail() { return 0; }
+};
+template <class L>
+struct OpNs {
+  static int run(const Ctx&) { return L::Ns; }
+  static int fail() { return -1; }
+};
+template <class L>
+struct OpEntropy {
+  static double run(const Ctx& c, const double* q) { return entropyintegral_t<L>(c, q); }
+  static double fail() { return NAN; }
+};
+template <class L>
+struct OpEntropyProduct {
+  static double run(const Ctx& c, const double* p, const double* q) { return entropyproduct_t<L>(c, p, q); }
+  static double fail() { return NAN; }
+};
+
+int nstates(const Ctx& c) { return dispatch<OpNs>(c); }
+
+int check(const orc_dg* o) {
+  if (!o) return -1;
+  Ctx c = make_ctx(o);
+  if (c.dim < 1 || c.dim > 3 || c.Nq < 2 || c.K < 0 || c.naux > MAXAUX) return -2;
+  if (nstates(c) < 0) return -3;
+  if (o->volume_form == 1 && o->fd_kernel == 2 && o->dirmask != (1 << c.dim) - 1) return -4;
+  if (!dispatch<OpSupported>(c)) return -5;
+  return 0;
+}
+
+void tendency(const orc_dg* o, const double* q, double* dq, bool inc) {
+  Ctx c = make_ctx(o);
+  dispatch<OpTendency>(c, q, dq, inc);
+}
+
+// ---------------------------------------------------------------------------
+// Low-storage RK tables: src/odesolvers.jl:148-229
+struct LSRKTable {
+  std::vector<double> a, b, c;
+};
+
+LSRKTable lsrk_table(int scheme) {
+  LSRKTable t;
+  if (scheme == 0) {
+    // rationals converted as Float64(num) / Float64(den)
+    t.a = {0.0, -567301805773.0 / 1357537059087.0, -2404267990393.0 / 2016746695238.0,
+           -3550918686646.0 / 2091501179385.0, -1275806237668.0 / 842570457699.0};
+    t.b = {1432997174477.0 / 9575080441755.0, 5161836677717.0 / 13612068292357.0,
+           1720146321549.0 / 2090206949498.0, 3134564353537.0 / 4481467310338.0,
+           2277821191437.0 / 14882151754819.0};
+    t.c = {0.0, 1432997174477.0 / 9575080441755.0, 2526269341429.0 / 6820363962896.0,
+           2006345519317.0 / 3224310063776.0, 2802321613138.0 / 2924317926251.0};
+  } else {
+    t.a = {0, -0.7188012108672410, -0.7785331173421570, -0.0053282796654044,
+           -0.8552979934029281, -3.9564138245774565, -1.5780575380587385,
+           -2.0837094552574054, -0.7483334182761610, -0.7032861106563359,
+           0.0013917096117681, -0.0932075369637460, -0.9514200470875948,
+           -7.1151571693922548};
+    t.b = {0.0367762454319673, 0.3136296607553959, 0.1531848691869027, 0.0030097086818182,
+           0.3326293790646110, 0.2440251405350864, 0.3718879239592277, 0.6204126221582444,
+           0.1524043173028741, 0.0760894927419266, 0.0077604214040978, 0.0024647284755382,
+           0.0780348340049386, 5.5059777270269628};
+    t.c = {0, 0.0367762454319673, 0.1249685262725025, 0.2446177702277698,
+           0.2476149531070420, 0.2969311120382472, 0.3978149645802642, 0.5270854589440328,
+           0.6981269994175695, 0.8190890835352128, 0.8527059887098624, 0.8604711817462826,
+           0.8627060376969976, 0.8734213127600976};
+  }
+  return t;
+}
+
+// src/odesolvers.jl:47-57
+void lsrk_dostep(const orc_dg* o, const LSRKTable& t, double* q, double* dq, int64_t n, double dt) {
+  for (size_t st = 0; st < t.a.size(); ++st) {
+    double a = t.a[st];
+    for (int64_t i = 0; i < n; ++i) dq[i] *= a;
+    tendency(o, q, dq, true);
+    double bdt = t.b[st] * dt;
+    for (int64_t i = 0; i < n; ++i) q[i] += bdt * dq[i];
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Banded vertical operator (src/bandedsystem.jl:1-15 + Bennu banded routines).
+// Unknowns of one vertical node line (fixed horizontal node position inside
+// one element column) are ordered m = s + Ns*(k + Nq*ez): the state fastest,
+// then the vertical node, then the element in the column.  Half bandwidth
+// bw = Ns*Nq - 1.  Storage: band[line][row][bw + col - row].
+struct Lines {
+  int Ns, Nq, Nh, KV, ncols, n, bw, nb;
+  int64_t nlines;
+  const int32_t* colelem;
+  int64_t node(int64_t line, int k, int ez, int Np) const {
+    int64_t c = line / Nh;
+    int h = (int)(line % Nh);
+    int64_t e = colelem[c * KV + ez];
+    return (int64_t)h + (int64_t)Nh * k + (int64_t)Np * e;
+  }
+};
+
+Lines make_lines(const Ctx& c, int Ns, int KV, const int32_t* colelem, int ncols) {
+  Lines L;
+  L.Ns = Ns;
+  L.Nq = c.Nq;
+  L.Nh = c.Np / c.Nq;
+  L.KV = KV;
+  L.ncols = ncols;
+  L.n = Ns * c.Nq * KV;
+  L.bw = Ns * c.Nq - 1;
+  L.nb = 2 * L.bw + 1;
+  L.nlines = (int64_t)ncols * L.Nh;
+  L.colelem = colelem;
+  return L;
+}
+
+// Probe the linear operator with coloured unit vectors (increment = false).
+void banded_assemble(const orc_dg* lin, const Lines& L, double* band) {
+  Ctx c = make_ctx(lin);
+  int64_t Nt = c.Ntot;
+  std::vector<double> q((size_t)L.Ns * Nt), dq((size_t)L.Ns * Nt);
+  std::fill(band, band + (size_t)L.nlines * L.n * L.nb, 0.0);
+  for (int color = 0; color < L.nb; ++color) {
+    std::fill(q.begin(), q.end(), 0.0);
+    for (int64_t ln = 0; ln < L.nlines; ++ln)
+      for (int m = color; m < L.n; m += L.nb) {
+        int s = m % L.Ns, k = (m / L.Ns) % L.Nq, ez = m / (L.Ns * L.Nq);
+        q[(size_t)s * Nt + L.node(ln, k, ez, c.Np)] = 1.0;
+      }
+    tendency(lin, q.data(), dq.data(), false);
+    for (int64_t ln = 0; ln < L.nlines; ++ln)
+      for (int r = 0; r < L.n; ++r) {
+        // the unique column of this colour inside the band of row r
+        int lo = std::max(0, r - L.bw);
+        int m = lo + ((color - lo) % L.nb + L.nb) % L.nb;
+        if (m > std::min(L.n - 1, r + L.bw)) continue;
+        int s = r % L.Ns, k = (r / L.Ns) % L.Nq, ez = r / (L.Ns * L.Nq);
+        band[((size_t)ln * L.n + r) * L.nb + (L.bw + m - r)] =
+            dq[(size_t)s * Nt + L.node(ln, k, ez, c.Np)];
+      }
+  }
+}
+
+// In-place banded LU without pivoting (Doolittle, unit lower triangle).
+void banded_lu(const Lines& L, double* band) {
+  const int n = L.n, bw = L.bw, nb = L.nb;
+#pragma omp parallel for schedule(static)
+  for (int64_t ln = 0; ln < L.nlines; ++ln) {
+    double* A = band + (size_t)ln * n * nb;
+    auto at = [&](int r, int c) -> double& { return A[(size_t)r * nb + (bw + c - r)]; };
+    for (int k = 0; k < n; ++k) {
+      double piv = at(k, k);
+      int iend = std::min(n - 1, k + bw);
+      for (int i = k + 1; i <= iend; ++i) {
+        double l = at(i, k) / piv;
+        at(i, k) = l;
+        for (int j = k + 1; j <= std::min(n - 1, k + bw); ++j) at(i, j) -= l * at(k, j);
+      }
+    }
+  }
+}
+
+void banded_solve(const Ctx& c, const Lines& L, const double* band, double* x, const double* b) {
+  const int n = L.n, bw = L.bw, nb = L.nb;
+#pragma omp parallel
+  {
+    std::vector<double> y(n);
+#pragma omp for schedule(static)
+    for (int64_t ln = 0; ln < L.nlines; ++ln) {
+      const double* A = band + (size_t)ln * n * nb;
+      auto at = [&](int r, int cc) { return A[(size_t)r * nb + (bw + cc - r)]; };
+      for (int r = 0; r < n; ++r) {
+        int s = r % L.Ns, k = (r / L.Ns) % L.Nq, ez = r / (L.Ns * L.Nq);
+        y[r] = b[(size_t)s * c.Ntot + L.node(ln, k, ez, c.Np)];
+      }
+      for (int r = 0; r < n; ++r) {
+        double v = y[r];
+        for (int cc = std::max(0, r - bw); cc < r; ++cc) v -= at(r, cc) * y[cc];
+        y[r] = v;
+      }
+      for (int r = n - 1; r >= 0; --r) {
+        double v = y[r];
+        for (int cc = r + 1; cc <= std::min(n - 1, r + bw); ++cc) v -= at(r, cc) * y[cc];
+        y[r] = v / at(r, r);
+      }
+      for (int r = 0; r < n; ++r) {
+        int s = r % L.Ns, k = (r / L.Ns) % L.Nq, ez = r / (L.Ns * L.Nq);
+        x[(size_t)s * c.Ntot + L.node(ln, k, ez, c.Np)] = y[r];
+      }
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// LGL operators (Bennu LobattoCell), long-double Newton on P_N'.
+void legendre(long double x, int N, long double& P, long double& dP) {
+  long double p0 = 1, p1 = x;
+  if (N == 0) { P = 1; dP = 0; return; }
+  for (int k = 2; k <= N; ++k) {
+    long double p2 = ((2 * k - 1) * x * p1 - (k - 1) * p0) / k;
+    p0 = p1;
+    p1 = p2;
+  }
+  P = p1;
+  dP = N * (p0 - x * p1) / (1 - x * x);
+}
+
+void lgl(int N, std::vector<long double>& x, std::vector<long double>& w) {
+  x.assign(N + 1, 0);
+  w.assign(N + 1, 0);
+  x[0] = -1;
+  x[N] = 1;
+  for (int i = 1; i < N; ++i) {
+    long double xi = -std::cos((long double)M_PI * i / N);
+    for (int it = 0; it < 100; ++it) {
+      // Newton on q(x) = P_N'(x) using q'(x) = (2x P_N' - N(N+1) P_N) / (1 - x^2)
+      long double P, dP;
+      legendre(xi, N, P, dP);
+      long double d2P = (2 * xi * dP - N * (N + 1) * P) / (1 - xi * xi);
+      long double dx = dP / d2P;
+      xi -= dx;
+      if (std::fabs(dx) < 1e-19L) break;
+    }
+    x[i] = xi;
+  }
+  for (int i = 0; i <= N; ++i) {
+    long double P, dP;
+    legendre(x[i], N, P, dP);
+    if (i == 0 || i == N) {
+      P = (i == 0 && N % 2 == 1) ? -1 : 1;
+    }
+    w[i] = 2.0L / (N * (N + 1) * P * P);
+  }
+}
+
+}  // namespace
+
+// ===========================================================================
+extern "C" {
+
+int orc_version() { return 1; }
+
+int orc_check(const orc_dg* o) { return check(o); }
+
+int orc_nstates(const orc_dg* o) {
+  Ctx c = make_ctx(o);
+  return nstates(c);
+}
+
+int orc_set_threads(int n) {
+  if (n > 0) omp_set_num_threads(n);
+  return omp_get_max_threads();
+}
+
+// LGL nodes/weights and derivative matrix D[r + Nq*c] = l_c'(xi_r).
+int orc_lgl(int N, double* xi, double* w, double* D) {
+  std::vector<long double> x, ww;
+  lgl(N, x, ww);
+  int Nq = N + 1;
+  for (int i = 0; i < Nq; ++i) { xi[i] = (double)x[i]; w[i] = (double)ww[i]; }
+  for (int r = 0; r < Nq; ++r) {
+    long double Pr, dPr;
+    legendre(x[r], N, Pr, dPr);
+    if (r == 0 || r == N) Pr = (r == 0 && N % 2 == 1) ? -1 : 1;
+    for (int c = 0; c < Nq; ++c) {
+      long double v;
+      if (r == c) {
+        v = (r == 0) ? -(long double)N * (N + 1) / 4 : (r == N ? (long double)N * (N + 1) / 4 : 0);
+      } else {
+        long double Pc, dPc;
+        legendre(x[c], N, Pc, dPc);
+        if (c == 0 || c == N) Pc = (c == 0 && N % 2 == 1) ? -1 : 1;
+        v = Pr / (Pc * (x[r] - x[c]));
+      }
+      D[r + Nq * c] = (double)v;
+    }
+  }
+  return 0;
+}
+
+// Brick grid (Bennu brickgrid): element e of multi-index (e0,e1,e2) is
+//   Cartesian: e = e0 + K0*(e1 + K1*e2)
+//   stacked:   e = e_last + K_last * (Cartesian index of the other dims)
+// Node ijk = i + Nq*(j + Nq*k); faces 2r+side (side 0: xi_r = -1).
+// Face point order: the other node indices, lower dimension fastest.
+// Boundary tag of a non-periodic domain face = 2r + side + 1.
+int orc_brickgrid(int dim, int N, const int32_t* Kd, const double* coords,
+                  const int32_t* periodic, int stacked, double* points, double* g,
+                  double* MJ, double* MJI, double* faceMJ, double* normal,
+                  int64_t* faceix_m, int64_t* faceix_p, int32_t* bctag, double* Dout) {
+  int Nq = N + 1;
+  std::vector<double> xi(Nq), w(Nq);
+  std::vector<double> Dm(Nq * Nq);
+  orc_lgl(N, xi.data(), w.data(), Dm.data());
+  if (Dout) std::memcpy(Dout, Dm.data(), sizeof(double) * Nq * Nq);
+  int K[3] = {1, 1, 1};
+  for (int d = 0; d < dim; ++d) K[d] = Kd[d];
+  const double* v[3];
+  int off = 0;
+  for (int d = 0; d < dim; ++d) { v[d] = coords + off; off += K[d] + 1; }
+  int Np = 1;
+  for (int d = 0; d < dim; ++d) Np *= Nq;
+  int Nfpf = Np / Nq, Nfp = 2 * dim * Nfpf;
+  int64_t Ktot = (int64_t)K[0] * K[1] * K[2];
+  int64_t Ntot = Ktot * Np, NF = Ktot * Nfp;
+  auto elem = [&](const int* em) -> int64_t {
+    if (!stacked) return em[0] + (int64_t)K[0] * (em[1] + (int64_t)K[1] * em[2]);
+    int V = dim - 1;
+    int64_t h = 0, stride = 1;
+    for (int d = 0; d < dim; ++d)
+      if (d != V) { h += em[d] * stride; stride *= K[d]; }
+    return em[V] + (int64_t)K[V] * h;
+  };
+  int em[3];
+  for (em[2] = 0; em[2] < K[2]; ++em[2])
+    for (em[1] = 0; em[1] < K[1]; ++em[1])
+      for (em[0] = 0; em[0] < K[0]; ++em[0]) {
+        int64_t e = elem(em);
+        double h[3] = {1, 1, 1};
+        for (int d = 0; d < dim; ++d) h[d] = v[d][em[d] + 1] - v[d][em[d]];
+        double J = 1;
+        for (int d = 0; d < dim; ++d) J *= h[d] / 2;
+        for (int ijk = 0; ijk < Np; ++ijk) {
+          int id[3] = {ijk % Nq, (ijk / Nq) % Nq, ijk / (Nq * Nq)};
+          int64_t node = ijk + (int64_t)Np * e;
+          double wp = 1;
+          for (int d = 0; d < dim; ++d) {
+            double a = v[d][em[d]], b = v[d][em[d] + 1];
+            points[d * Ntot + node] = (1 - xi[id[d]]) / 2 * a + (1 + xi[id[d]]) / 2 * b;
+            wp *= w[id[d]];
+          }
+          for (int a = 0; a < dim; ++a)
+            for (int b = 0; b < dim; ++b) g[(a + dim * b) * Ntot + node] = (a == b) ? 2 / h[a] : 0.0;
+          MJ[node] = wp * J;
+          MJI[node] = 1 / MJ[node];
+        }
+        for (int r = 0; r < dim; ++r)
+          for (int side = 0; side < 2; ++side) {
+            int face = 2 * r + side;
+            int nb[3] = {em[0], em[1], em[2]};
+            nb[r] += side == 0 ? -1 : 1;
+            bool bnd = false;
+            if (nb[r] < 0 || nb[r] >= K[r]) {
+              if (periodic[r]) nb[r] = (nb[r] + K[r]) % K[r];
+              else bnd = true;
+            }
+            int64_t en = bnd ? e : elem(nb);
+            bctag[face + 2 * dim * e] = bnd ? face + 1 : 0;
+            double fJ = 1;
+            for (int d = 0; d < dim; ++d)
+              if (d != r) fJ *= h[d] / 2;
+            for (int fp = 0; fp < Nfpf; ++fp) {
+              // decode the face point into the node indices of the other dims
+              int id[3] = {0, 0, 0};
+              int rem = fp;
+              double wf = 1;
+              for (int d = 0; d < dim; ++d) {
+                if (d == r) continue;
+                id[d] = rem % Nq;
+                rem /= Nq;
+                wf *= w[id[d]];
+              }
+              int idm[3] = {id[0], id[1], id[2]}, idp[3] = {id[0], id[1], id[2]};
+              idm[r] = side == 0 ? 0 : Nq - 1;
+              idp[r] = side == 0 ? Nq - 1 : 0;
+              int64_t jj = face * Nfpf + fp + (int64_t)Nfp * e;
+              int64_t nm = idm[0] + Nq * (idm[1] + (int64_t)Nq * idm[2]);
+              int64_t npp = idp[0] + Nq * (idp[1] + (int64_t)Nq * idp[2]);
+              faceix_m[jj] = nm + (int64_t)Np * e;
+              faceix_p[jj] = bnd ? faceix_m[jj] : npp + (int64_t)Np * en;
+              faceMJ[jj] = wf * fJ;
+              for (int d = 0; d < dim; ++d)
+                normal[d * NF + jj] = (d == r) ? (side == 0 ? -1.0 : 1.0) : 0.0;
+            }
+          }
+      }
+  return 0;
+}
+
+int orc_tendency(const orc_dg* o, const double* q, double* dq, int increment) {
+  int rc = check(o);
+  if (rc) return rc;
+  tendency(o, q, dq, increment != 0);
+  return 0;
+}
+
+// solve! with an LSRK scheme (0 = LSRK54, 1 = LSRK144): src/odesolvers.jl:9-26, :47-57
+int orc_solve_lsrk(const orc_dg* o, double* q, double dt, double t0, double timeend,
+                   int scheme, int adjust_final, int64_t max_steps, int64_t* nsteps_out,
+                   double* time_out) {
+  int rc = check(o);
+  if (rc) return rc;
+  Ctx c = make_ctx(o);
+  int Ns = nstates(c);
+  int64_t n = (int64_t)Ns * c.Ntot;
+  std::vector<double> dq(n, 0.0);
+  LSRKTable t = lsrk_table(scheme);
+  double time = t0;
+  int64_t step = 0;
+  while (true) {
+    ++step;
+    bool finalstep = false;
+    if (time + dt >= timeend) {
+      if (adjust_final) dt = timeend - time;
+      finalstep = true;
+    }
+    lsrk_dostep(o, t, q, dq.data(), n, dt);
+    time += dt;
+    if (finalstep || (max_steps > 0 && step >= max_steps)) break;
+  }
+  if (nsteps_out) *nsteps_out = step;
+  if (time_out) *time_out = time;
+  return 0;
+}
+
+// A fixed number of LSRK steps with constant dt; dq is caller-owned (so that a
+// caller can continue a run) and must start at zero.
+int orc_lsrk_steps(const orc_dg* o, double* q, double* dq, double dt, int64_t nsteps, int scheme) {
+  int rc = check(o);
+  if (rc) return rc;
+  Ctx c = make_ctx(o);
+  int64_t n = (int64_t)nstates(c) * c.Ntot;
+  LSRKTable t = lsrk_table(scheme);
+  for (int64_t s = 0; s < nsteps; ++s) lsrk_dostep(o, t, q, dq, n, dt);
+  return 0;
+}
+
+// One fused LSRK stage as two explicit steps: dq = a*dq + L(q); q += bdt*dq.
+int orc_lsrk_stage(const orc_dg* o, double* q, double* dq, double a, double bdt) {
+  int rc = check(o);
+  if (rc) return rc;
+  Ctx c = make_ctx(o);
+  int64_t n = (int64_t)nstates(c) * c.Ntot;
+  for (int64_t i = 0; i < n; ++i) dq[i] *= a;
+  tendency(o, q, dq, true);
+  for (int64_t i = 0; i < n; ++i) q[i] += bdt * dq[i];
+  return 0;
+}
+
+// src/measures.jl:6-14 (p = 2, not componentwise)
+double orc_weightednorm(int Ns, int64_t Ntot, const double* MJ, const double* q) {
+  long double tot = 0;
+  for (int s = 0; s < Ns; ++s) {
+    long double acc = 0;
+    for (int64_t i = 0; i < Ntot; ++i) {
+      double a = std::fabs(q[s * Ntot + i]);
+      acc += MJ[i] * (a * a);
+    }
+    double ns = std::sqrt((double)acc);
+    tot += (long double)ns * ns;
+  }
+  return std::sqrt((double)tot);
+}
+
+double orc_entropyintegral(const orc_dg* o, const double* q) {
+  Ctx c = make_ctx(o);
+  return dispatch<OpEntropy>(c, q);
+}
+
+double orc_entropyproduct(const orc_dg* o, const double* p, const double* q) {
+  Ctx c = make_ctx(o);
+  return dispatch<OpEntropyProduct>(c, p, q);
+}
+
+// solve! with RLSRK (relaxation LSRK, src/odesolvers.jl:69-136).  The scalar
+// relaxation equation is solved by Newton's method with the analytic
+// derivative, from the previous step's gamma, stopping at |r| <= 10 eps.
+int orc_solve_rlsrk(const orc_dg* o, double* q, double dt, double t0, double timeend,
+                    int scheme, int adjust_final, int64_t* nsteps_out, double* gamma_out) {
+  int rc = check(o);
+  if (rc) return rc;
+  Ctx c = make_ctx(o);
+  int Ns = nstates(c);
+  int64_t n = (int64_t)Ns * c.Ntot;
+  std::vector<double> dq(n, 0.0), q0(n), k(n);
+  LSRKTable t = lsrk_table(scheme);
+  int NS = (int)t.a.size();
+  std::vector<double> bfull(NS);
+  bfull[NS - 1] = t.b[NS - 1];
+  for (int i = NS - 2; i >= 0; --i) bfull[i] = t.a[i + 1] * bfull[i + 1] + t.b[i];
+  double gam = 1.0, time = t0;
+  int64_t step = 0;
+  const double ftol = 10 * DBL_EPSILON;
+  while (true) {
+    ++step;
+    bool finalstep = false;
+    if (time + dt >= timeend) {
+      if (adjust_final) dt = timeend - time;
+      finalstep = true;
+    }
+    std::copy(q, q + n, q0.begin());
+    double eta0 = orc_entropyintegral(o, q0.data());
+    double deta = -0.0;
+    for (int st = 0; st < NS; ++st) {
+      for (int64_t i = 0; i < n; ++i) dq[i] *= t.a[st];
+      tendency(o, q, k.data(), false);
+      for (int64_t i = 0; i < n; ++i) dq[i] += k[i];
+      deta += bfull[st] * dt * orc_entropyproduct(o, q, k.data());
+      double bdt = t.b[st] * dt;
+      for (int64_t i = 0; i < n; ++i) q[i] += bdt * dq[i];
+    }
+    for (int64_t i = 0; i < n; ++i) k[i] = q[i] - q0[i];
+    auto setq = [&](double gg) {
+      for (int64_t i = 0; i < n; ++i) q[i] = q0[i] + gg * k[i];
+    };
+    // r(0) = 0 is always a (trivial) root; the iteration must stay on the
+    // root near 1, so it stops once the Newton step is at round-off level.
+    for (int it = 0; it < 50; ++it) {
+      setq(gam);
+      double r = orc_entropyintegral(o, q) - eta0 - gam * deta;
+      if (std::fabs(r) <= ftol) break;
+      double J = orc_entropyproduct(o, q, k.data()) - deta;
+      double step_ = r / J;
+      if (!std::isfinite(step_)) break;
+      gam -= step_;
+      if (std::fabs(step_) <= 4 * DBL_EPSILON * std::fabs(gam)) break;
+    }
+    setq(gam);
+    time += dt;
+    if (finalstep) break;
+  }
+  if (nsteps_out) *nsteps_out = step;
+  if (gamma_out) *gamma_out = gam;
+  return 0;
+}
+
+int orc_band_shape(const orc_dg* lin, int KV, int ncols, int32_t* n_out, int32_t* bw_out,
+                   int64_t* nlines_out) {
+  Ctx c = make_ctx(lin);
+  Lines L = make_lines(c, nstates(c), KV, nullptr, ncols);
+  *n_out = L.n;
+  *bw_out = L.bw;
+  *nlines_out = L.nlines;
+  return 0;
+}
+
+// Assemble the probed band of L (before scaling) into band[line][row][bw+col-row].
+int orc_banded_assemble(const orc_dg* lin, int KV, const int32_t* colelem, int ncols, double* band) {
+  int rc = check(lin);
+  if (rc) return rc;
+  Ctx c = make_ctx(lin);
+  Lines L = make_lines(c, nstates(c), KV, colelem, ncols);
+  banded_assemble(lin, L, band);
+  return 0;
+}
+
+// A = I + scale*L (scale = -dt*gamma_ii), factored in place.
+int orc_banded_factor(const orc_dg* lin, int KV, int ncols, double* band, double scale) {
+  Ctx c = make_ctx(lin);
+  Lines L = make_lines(c, nstates(c), KV, nullptr, ncols);
+  size_t tot = (size_t)L.nlines * L.n * L.nb;
+  for (size_t i = 0; i < tot; ++i) band[i] *= scale;
+  for (int64_t ln = 0; ln < L.nlines; ++ln)
+    for (int r = 0; r < L.n; ++r) band[((size_t)ln * L.n + r) * L.nb + L.bw] += 1;
+  banded_lu(L, band);
+  return 0;
+}
+
+int orc_banded_solve(const orc_dg* lin, int KV, const int32_t* colelem, int ncols,
+                     const double* band, double* x, const double* b) {
+  Ctx c = make_ctx(lin);
+  Lines L = make_lines(c, nstates(c), KV, colelem, ncols);
+  banded_solve(c, L, band, x, b);
+  return 0;
+}
+
+// solve! with ARK23 (src/odesolvers.jl:231-388).  `ex` may be NULL (no
+// explicit operator); `lin` is the vertical linear operator, probed into a
+// banded matrix and factored once with gamma = im_rka[3,3].
+int orc_solve_ark23(const orc_dg* ex, const orc_dg* lin, double* q, double dt, double t0,
+                    double timeend, int split_rhs, int paperversion, int adjust_final,
+                    int KV, const int32_t* colelem, int ncols, int64_t max_steps,
+                    int64_t* nsteps_out, double* time_out) {
+  int rc = check(lin);
+  if (rc) return rc;
+  if (ex && (rc = check(ex))) return rc;
+  Ctx c = make_ctx(lin);
+  int Ns = nstates(c);
+  int64_t n = (int64_t)Ns * c.Ntot;
+  const double s2 = std::sqrt(2.0);
+  double a32 = paperversion ? (3 + 2 * s2) / 6 : 0.5;
+  double exa[3][3] = {{0, 0, 0}, {2 - s2, 0, 0}, {1 - a32, a32, 0}};
+  double exb[3] = {1 / (2 * s2), 1 / (2 * s2), 1 - 1 / s2};
+  double ima[3][3] = {{0, 0, 0}, {1 - 1 / s2, 1 - 1 / s2, 0}, {1 / (2 * s2), 1 / (2 * s2), 1 - 1 / s2}};
+  double imb[3] = {exb[0], exb[1], exb[2]};
+  Lines L = make_lines(c, Ns, KV, colelem, ncols);
+  std::vector<double> band((size_t)L.nlines * L.n * L.nb);
+  banded_assemble(lin, L, band.data());
+  orc_banded_factor(lin, KV, ncols, band.data(), -dt * ima[2][2]);
+  std::vector<double> Qhat(n), exK(3 * n), imK(3 * n), Qs(2 * n);
+  double time = t0;
+  int64_t step = 0;
+  while (true) {
+    ++step;
+    bool finalstep = false;
+    if (time + dt >= timeend) {
+      if (adjust_final) dt = timeend - time;  // (the factorisation keeps its dt, as in the reference)
+      finalstep = true;
+    }
+    double* Q[3] = {q, Qs.data(), Qs.data() + n};
+    for (int i = 0; i < 3; ++i) {
+      if (i > 0) {
+        for (int64_t m = 0; m < n; ++m) Qhat[m] = q[m];
+        for (int kk = 0; kk < i; ++kk) {
+          double ea = exa[i][kk], ia = ima[i][kk];
+          const double* eK = exK.data() + kk * n;
+          const double* iK = imK.data() + kk * n;
+          for (int64_t m = 0; m < n; ++m) Qhat[m] += dt * (ea * eK[m] + ia * iK[m]);
+        }
+        banded_solve(c, L, band.data(), Q[i], Qhat.data());
+      }
+      double* eK = exK.data() + i * n;
+      double* iK = imK.data() + i * n;
+      if (ex) tendency(ex, Q[i], eK, false);
+      else std::fill(eK, eK + n, 0.0);
+      tendency(lin, Q[i], iK, false);
+      if (!split_rhs)
+        for (int64_t m = 0; m < n; ++m) eK[m] -= iK[m];
+    }
+    for (int i = 0; i < 3; ++i) {
+      const double* eK = exK.data() + i * n;
+      const double* iK = imK.data() + i * n;
+      for (int64_t m = 0; m < n; ++m) q[m] += dt * (exb[i] * eK[m] + imb[i] * iK[m]);
+    }
+    time += dt;
+    if (finalstep || (max_steps > 0 && step >= max_steps)) break;
+  }
+  if (nsteps_out) *nsteps_out = step;
+  if (time_out) *time_out = time;
+  return 0;
+}
+
+}  // extern "C"
