@@ -1,0 +1,146 @@
+"""DGSEM operator: the drop-in for `DGSEM(...)` and its `rhs!` protocol.
+
+`DGSEM(law=..., grid=..., surface_numericalflux=..., volume_form=...,
+directions=..., auxstate=...)` mirrors src/dgsem.jl:38-56; calling the object
+as `dg(dq, q, time, increment=True)` mirrors src/dgsem.jl:59-98 and runs the
+fused volume + surface kernel of libatumhip.so on the device fields.
+"""
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+
+from . import _lib
+from .fields import Field
+from .laws import AbstractNumericalFlux, EntropyConservativeFlux
+
+
+class WeakForm:
+    """src/dgsem.jl:7."""
+
+    def __repr__(self):
+        return "WeakForm()"
+
+    def __eq__(self, other):
+        return isinstance(other, WeakForm)
+
+    def __hash__(self):
+        return hash("WeakForm")
+
+
+class FluxDifferencingForm:
+    """src/dgsem.jl:8-14.  All three reference kernel variants (:per_dir,
+    :per_dir_symmetric, :naive) compute the same sum; the device kernel keeps
+    the :per_dir order."""
+
+    def __init__(self, volume_numericalflux: AbstractNumericalFlux = EntropyConservativeFlux(),
+                 kernel_type: str = "per_dir"):
+        if kernel_type not in ("per_dir", "per_dir_symmetric", "naive"):
+            raise ValueError(f"Unknown kernel type {kernel_type}")
+        self.volume_numericalflux = volume_numericalflux
+        self.kernel_type = kernel_type
+
+    def __repr__(self):
+        return f"FluxDifferencingForm({self.volume_numericalflux!r})"
+
+    def __eq__(self, other):
+        return (isinstance(other, FluxDifferencingForm)
+                and other.volume_numericalflux == self.volume_numericalflux)
+
+    def __hash__(self):
+        return hash(("FD", self.volume_numericalflux))
+
+
+class DGSEM:
+    def __init__(self, *, law, grid, surface_numericalflux, volume_form=None, directions=None,
+                 auxstate=None, ctx=None):
+        self.ctx = ctx or _lib.context()
+        self.law = law
+        self.grid = grid
+        self.volume_form = volume_form if volume_form is not None else WeakForm()
+        self.surface_numericalflux = surface_numericalflux
+        dim = grid.dim
+        if law.dim != dim:
+            raise ValueError("law and grid dimensions differ")
+        self.directions = tuple(directions) if directions is not None else tuple(range(1, dim + 1))
+        if not self.directions or any(d < 1 or d > dim for d in self.directions):
+            raise ValueError("directions must be a non-empty subset of 1:dim")
+        # src/dgsem.jl:41-51
+        self.MJ = grid.MJ
+        self.MJI = grid.MJI
+        self.faceMJ = grid.faceMJ
+        if auxstate is None:
+            auxstate = law.auxiliary(grid.points)
+        self.auxstate = None if auxstate is None else np.ascontiguousarray(auxstate, dtype=np.float64)
+        naux = 0 if self.auxstate is None else self.auxstate.shape[0]
+        prob = law.problem
+        d = _lib.DgDesc()
+        d.law = _lib.LAWS[law.law]
+        d.dim, d.N, d.K = dim, grid.N, grid.K
+        if isinstance(self.volume_form, WeakForm):
+            d.volume_form, d.volume_flux = 0, 0
+        else:
+            d.volume_form = 1
+            d.volume_flux = _lib.FLUX_IDS[self.volume_form.volume_numericalflux.name]
+        d.surface_flux = _lib.FLUX_IDS[surface_numericalflux.name]
+        d.dirmask = sum(1 << (r - 1) for r in self.directions)
+        d.bc = _lib.BC_IDS[prob.boundary]
+        d.source = _lib.SOURCE_IDS[prob.source]
+        d.pde_level_balance = int(getattr(law, "pde_level_balance", False))
+        d.naux = naux
+        d.gamma = float(getattr(law, "gamma", 1.4))
+        d.grav = float(getattr(law, "grav", 9.81))
+        d.omega = float(prob.omega)
+        u = getattr(law, "u", (0.0,) * dim)
+        for i in range(3):
+            d.u[i] = float(u[i]) if i < len(u) else 0.0
+        keep = []
+
+        def ptr(a, t=ctypes.c_double):
+            a = np.ascontiguousarray(a)
+            keep.append(a)
+            return a.ctypes.data_as(ctypes.POINTER(t))
+
+        if prob.boundary == "prescribed":
+            st = np.ascontiguousarray(prob.bc_states, dtype=np.float64)
+            d.n_bc_states = st.shape[0]
+            d.bc_states = ptr(st)
+        d.D = ptr(grid.D.ravel(order="F"))
+        d.g, d.MJ, d.MJI = ptr(grid.g), ptr(grid.MJ), ptr(grid.MJI)
+        d.faceMJ, d.normal = ptr(grid.faceMJ), ptr(grid.normal)
+        if naux:
+            d.aux = ptr(self.auxstate)
+        d.faceix_m = ptr(grid.faceix_m, ctypes.c_int64)
+        d.faceix_p = ptr(grid.faceix_p, ctypes.c_int64)
+        d.bctag = ptr(grid.bctag, ctypes.c_int32)
+        h = ctypes.c_void_p()
+        self.ctx.check(_lib.lib.atm_dg_create(self.ctx.h, ctypes.byref(d), ctypes.byref(h)),
+                       "atm_dg_create")
+        self.h = h
+        self.Ns = _lib.lib.atm_dg_nstates(h)
+
+    def __call__(self, dq: Field, q: Field, time: float = 0.0, increment: bool = True):
+        self.ctx.check(_lib.lib.atm_dg_tendency(self.h, dq.ptr, q.ptr, float(time), int(increment)),
+                       "atm_dg_tendency")
+        return dq
+
+    def lsrk_stage(self, q_in: Field, q_out: Field, dq: Field, a: float, bdt: float, time=0.0):
+        self.ctx.check(_lib.lib.atm_lsrk_stage(self.h, q_in.ptr, q_out.ptr, dq.ptr, float(a),
+                                               float(bdt), float(time)), "atm_lsrk_stage")
+
+    def fieldarray(self, values=None):
+        f = Field(self.Ns, self.grid.Ntot, self.ctx)
+        if values is None:
+            f.fill_(0.0)
+        else:
+            f.upload(values)
+        return f
+
+    def __del__(self):
+        try:
+            if getattr(self, "h", None):
+                _lib.lib.atm_dg_destroy(self.h)
+                self.h = None
+        except Exception:
+            pass

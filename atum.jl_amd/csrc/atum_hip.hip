@@ -1,0 +1,787 @@
+// libatumhip.so: C-ABI implementation (contexts, device memory, DGSEM operator,
+// fused LSRK driver with hipGraph replay, banded vertical solver, ARK23).
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "resolve.hpp"
+
+using namespace atm;
+
+struct atm_ctx {
+  int device = 0;
+  hipStream_t own = nullptr;
+  hipStream_t stream = nullptr;
+  size_t max_lds = 65536;
+  std::string err;
+  hipEvent_t events[16] = {};
+};
+
+struct atm_dg {
+  atm_ctx* ctx = nullptr;
+  DevDG dev{};
+  LaunchFn fn = nullptr;
+  int law = 0, dim = 0, Nq = 0, Np = 0, Ns = 0;
+  int64_t Ntot = 0;
+  std::vector<void*> allocs;
+};
+
+struct atm_banded {
+  atm_dg* lin = nullptr;
+  int n = 0, bw = 0, nb = 0, KV = 0, ncols = 0, Nh = 0;
+  int64_t nlines = 0;
+  double* band = nullptr;      // [(r*nb + bw + c - r) * nlines + line]
+  int32_t* elem_col = nullptr;  // column of each element (-1: none)
+  int32_t* elem_ez = nullptr;   // position of each element in its column
+  int32_t* colelem = nullptr;   // ncols*KV
+};
+
+namespace {
+
+int fail(atm_ctx* ctx, int code, const std::string& msg) {
+  if (ctx) ctx->err = msg;
+  return code;
+}
+
+int hipfail(atm_ctx* ctx, hipError_t e, const char* what) {
+  return fail(ctx, ATM_ERR_HIP, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+#define HIPCHK(ctx, call)                               \
+  do {                                                  \
+    hipError_t _e = (call);                             \
+    if (_e != hipSuccess) return hipfail(ctx, _e, #call); \
+  } while (0)
+
+constexpr int EW_THREADS = 256;
+
+inline int ew_blocks(int64_t n) {
+  int64_t b = (n + EW_THREADS - 1) / EW_THREADS;
+  return (int)(b > 65535 * 16 ? 65535 * 16 : (b < 1 ? 1 : b));
+}
+
+__global__ void axpby_kernel(double* __restrict__ y, const double* __restrict__ x, double a, double b,
+                             int64_t n) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    y[i] = a * y[i] + b * x[i];
+}
+
+__global__ void fill_kernel(double* __restrict__ y, double v, int64_t n) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    y[i] = v;
+}
+
+__global__ void copy_kernel(double* __restrict__ y, const double* __restrict__ x, int64_t n) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    y[i] = x[i];
+}
+
+// y -= x
+__global__ void sub_kernel(double* __restrict__ y, const double* __restrict__ x, int64_t n) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    y[i] = y[i] - x[i];
+}
+
+struct ArkCoef {
+  double ea[3], ia[3];
+};
+
+// Qhat = q; Qhat += dt*(a_k*exK_k + ã_k*imK_k) for k < nk (src/odesolvers.jl:346-349)
+__global__ void ark_combine_kernel(double* out, const double* q, double dt,
+                                   int nk, ArkCoef c, const double* __restrict__ e0,
+                                   const double* __restrict__ e1, const double* __restrict__ e2,
+                                   const double* __restrict__ i0, const double* __restrict__ i1,
+                                   const double* __restrict__ i2, int64_t n) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    double v = q[i];
+    if (nk > 0) v = v + dt * (c.ea[0] * e0[i] + c.ia[0] * i0[i]);
+    if (nk > 1) v = v + dt * (c.ea[1] * e1[i] + c.ia[1] * i1[i]);
+    if (nk > 2) v = v + dt * (c.ea[2] * e2[i] + c.ia[2] * i2[i]);
+    out[i] = v;
+  }
+}
+
+// ---- banded -----------------------------------------------------------------
+struct BandGeo {
+  int Ns, Nq, Nh, Np, KV, n, bw, nb;
+  int64_t nlines, Ntot;
+  const int32_t* elem_col;
+  const int32_t* elem_ez;
+  const int32_t* colelem;
+};
+
+__device__ __forceinline__ int64_t band_node(const BandGeo& g, int64_t line, int r, int* s) {
+  const int64_t c = line / g.Nh;
+  const int h = (int)(line - c * g.Nh);
+  *s = r % g.Ns;
+  const int rem = r / g.Ns;
+  const int k = rem % g.Nq, ez = rem / g.Nq;
+  const int64_t e = g.colelem[c * g.KV + ez];
+  return (int64_t)h + (int64_t)g.Nh * k + (int64_t)g.Np * e;
+}
+
+// probe vector: 1 on every unknown whose in-line index m satisfies m % nb == color
+__global__ void band_probe_kernel(BandGeo g, double* __restrict__ q, int color) {
+  const int64_t tot = (int64_t)g.Ns * g.Ntot;
+  for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < tot; t += (int64_t)gridDim.x * blockDim.x) {
+    const int s = (int)(t / g.Ntot);
+    const int64_t node = t - (int64_t)s * g.Ntot;
+    const int64_t e = node / g.Np;
+    const int ijk = (int)(node - e * g.Np);
+    double v = 0.0;
+    if (g.elem_col[e] >= 0) {
+      const int k = ijk / g.Nh;
+      const int m = s + g.Ns * (k + g.Nq * g.elem_ez[e]);
+      v = (m % g.nb == color) ? 1.0 : 0.0;
+    }
+    q[t] = v;
+  }
+}
+
+__global__ void band_scatter_kernel(BandGeo g, const double* __restrict__ dq, double* __restrict__ band,
+                                    int color) {
+  const int64_t tot = g.nlines * g.n;
+  for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < tot; t += (int64_t)gridDim.x * blockDim.x) {
+    const int r = (int)(t / g.nlines);
+    const int64_t line = t - (int64_t)r * g.nlines;
+    const int lo = r - g.bw > 0 ? r - g.bw : 0;
+    const int m = lo + ((color - lo) % g.nb + g.nb) % g.nb;
+    const int hi = r + g.bw < g.n - 1 ? r + g.bw : g.n - 1;
+    if (m > hi) continue;
+    int s;
+    const int64_t node = band_node(g, line, r, &s);
+    band[((int64_t)r * g.nb + (g.bw + m - r)) * g.nlines + line] = dq[(int64_t)s * g.Ntot + node];
+  }
+}
+
+// A = I + scale*L, then in-place LU without pivoting (unit lower factor).
+__global__ void band_lu_kernel(BandGeo g, double* __restrict__ band, double scale) {
+  const int64_t line = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (line >= g.nlines) return;
+  const int n = g.n, bw = g.bw, nb = g.nb;
+  const int64_t L = g.nlines;
+  auto A = [&](int r, int c) -> double& { return band[((int64_t)r * nb + (bw + c - r)) * L + line]; };
+  for (int r = 0; r < n; ++r)
+    for (int c = (r - bw > 0 ? r - bw : 0); c <= (r + bw < n - 1 ? r + bw : n - 1); ++c) {
+      double v = A(r, c) * scale;
+      if (r == c) v += 1;
+      A(r, c) = v;
+    }
+  for (int k = 0; k < n; ++k) {
+    const double piv = A(k, k);
+    const int iend = k + bw < n - 1 ? k + bw : n - 1;
+    for (int i = k + 1; i <= iend; ++i) {
+      const double l = A(i, k) / piv;
+      A(i, k) = l;
+      for (int j = k + 1; j <= iend; ++j) A(i, j) -= l * A(k, j);
+    }
+  }
+}
+
+// Forward and back substitution, one thread per line; x may alias b.
+__global__ void band_solve_kernel(BandGeo g, const double* __restrict__ band, double* x, const double* b) {
+  const int64_t line = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (line >= g.nlines) return;
+  const int n = g.n, bw = g.bw, nb = g.nb;
+  const int64_t L = g.nlines;
+  auto A = [&](int r, int c) { return band[((int64_t)r * nb + (bw + c - r)) * L + line]; };
+  auto idx = [&](int r) {
+    int s;
+    const int64_t node = band_node(g, line, r, &s);
+    return (int64_t)s * g.Ntot + node;
+  };
+  for (int r = 0; r < n; ++r) {
+    double v = b[idx(r)];
+    for (int c = (r - bw > 0 ? r - bw : 0); c < r; ++c) v -= A(r, c) * x[idx(c)];
+    x[idx(r)] = v;
+  }
+  for (int r = n - 1; r >= 0; --r) {
+    double v = x[idx(r)];
+    for (int c = r + 1; c <= (r + bw < n - 1 ? r + bw : n - 1); ++c) v -= A(r, c) * x[idx(c)];
+    x[idx(r)] = v / A(r, r);
+  }
+}
+
+BandGeo band_geo(const atm_banded* b) {
+  BandGeo g;
+  g.Ns = b->lin->Ns;
+  g.Nq = b->lin->Nq;
+  g.Nh = b->Nh;
+  g.Np = b->lin->Np;
+  g.KV = b->KV;
+  g.n = b->n;
+  g.bw = b->bw;
+  g.nb = b->nb;
+  g.nlines = b->nlines;
+  g.Ntot = b->lin->Ntot;
+  g.elem_col = b->elem_col;
+  g.elem_ez = b->elem_ez;
+  g.colelem = b->colelem;
+  return g;
+}
+
+// ---- LSRK tables (src/odesolvers.jl:148-229) ---------------------------------
+struct LSRKTable {
+  int ns;
+  double a[14], b[14];
+};
+
+LSRKTable lsrk_table(int scheme) {
+  LSRKTable t{};
+  if (scheme == ATM_LSRK54) {
+    const double a[5] = {0.0, -567301805773.0 / 1357537059087.0, -2404267990393.0 / 2016746695238.0,
+                         -3550918686646.0 / 2091501179385.0, -1275806237668.0 / 842570457699.0};
+    const double b[5] = {1432997174477.0 / 9575080441755.0, 5161836677717.0 / 13612068292357.0,
+                         1720146321549.0 / 2090206949498.0, 3134564353537.0 / 4481467310338.0,
+                         2277821191437.0 / 14882151754819.0};
+    t.ns = 5;
+    for (int i = 0; i < 5; ++i) { t.a[i] = a[i]; t.b[i] = b[i]; }
+  } else {
+    const double a[14] = {0, -0.7188012108672410, -0.7785331173421570, -0.0053282796654044,
+                          -0.8552979934029281, -3.9564138245774565, -1.5780575380587385,
+                          -2.0837094552574054, -0.7483334182761610, -0.7032861106563359,
+                          0.0013917096117681, -0.0932075369637460, -0.9514200470875948,
+                          -7.1151571693922548};
+    const double b[14] = {0.0367762454319673, 0.3136296607553959, 0.1531848691869027,
+                          0.0030097086818182, 0.3326293790646110, 0.2440251405350864,
+                          0.3718879239592277, 0.6204126221582444, 0.1524043173028741,
+                          0.0760894927419266, 0.0077604214040978, 0.0024647284755382,
+                          0.0780348340049386, 5.5059777270269628};
+    t.ns = 14;
+    for (int i = 0; i < 14; ++i) { t.a[i] = a[i]; t.b[i] = b[i]; }
+  }
+  return t;
+}
+
+Resolved resolve(const atm_dg_desc* d, int Nq, int Np) {
+  const int f = d->volume_form, vf = d->volume_flux, sf = d->surface_flux;
+  switch (d->law) {
+    case ATM_LAW_ADVECTION: return resolve_advection(d->dim, f, vf, sf, Nq, Np);
+    case ATM_LAW_EULER: return resolve_euler(d->dim, f, vf, sf, Nq, Np);
+    case ATM_LAW_EULER_GRAVITY: return resolve_euler_gravity(d->dim, f, vf, sf, Nq, Np);
+    case ATM_LAW_LINEAR_EULER_GRAVITY: return resolve_linear_euler_gravity(d->dim, f, vf, sf, Nq, Np);
+    case ATM_LAW_SHALLOW_WATER: return resolve_shallow_water(d->dim, f, vf, sf, Nq, Np);
+    default: return Resolved{nullptr, 0, 0};
+  }
+}
+
+template <template <int> class LAW>
+bool sflux_ok_dims(int dim, int sf) {
+  if (dim == 1) return sflux_supported<LAW<1>>(sf);
+  if (dim == 2) return sflux_supported<LAW<2>>(sf);
+  return sflux_supported<LAW<3>>(sf);
+}
+
+bool sflux_ok(int law, int dim, int sf) {
+  switch (law) {
+    case ATM_LAW_ADVECTION: return sflux_ok_dims<Advection>(dim, sf);
+    case ATM_LAW_EULER: return sflux_ok_dims<Euler>(dim, sf);
+    case ATM_LAW_EULER_GRAVITY: return sflux_ok_dims<EulerGravity>(dim, sf);
+    case ATM_LAW_LINEAR_EULER_GRAVITY: return sflux_ok_dims<LinearEulerGravity>(dim, sf);
+    case ATM_LAW_SHALLOW_WATER: return sflux_ok_dims<ShallowWater>(dim, sf);
+    default: return false;
+  }
+}
+
+template <class T>
+int upload_copy(atm_dg* dg, const T* host, size_t count, const T** out) {
+  *out = nullptr;
+  if (count == 0) return ATM_OK;
+  if (!host) return fail(dg->ctx, ATM_ERR_ARG, "atm_dg_create: missing geometry array");
+  void* p = nullptr;
+  HIPCHK(dg->ctx, hipMalloc(&p, count * sizeof(T)));
+  dg->allocs.push_back(p);
+  HIPCHK(dg->ctx, hipMemcpy(p, host, count * sizeof(T), hipMemcpyHostToDevice));
+  *out = static_cast<const T*>(p);
+  return ATM_OK;
+}
+
+int launch(atm_dg* dg, const double* q, double* dq, double alpha, int assign, double* qout, double bdt) {
+  hipError_t e = dg->fn(dg->dev, dg->ctx->stream, q, dq, alpha, assign, qout, bdt);
+  if (e != hipSuccess) return hipfail(dg->ctx, e, "element kernel launch");
+  return ATM_OK;
+}
+
+}  // namespace
+
+// =============================================================================
+extern "C" {
+
+const char* atm_version(void) { return "atum-hip 0.1 gfx950"; }
+
+int atm_ctx_create(int device, atm_ctx** out) {
+  if (!out) return ATM_ERR_ARG;
+  *out = nullptr;
+  int count = 0;
+  if (hipGetDeviceCount(&count) != hipSuccess || device < 0 || device >= count) return ATM_ERR_HIP;
+  if (hipSetDevice(device) != hipSuccess) return ATM_ERR_HIP;
+  atm_ctx* c = new atm_ctx();
+  c->device = device;
+  if (hipStreamCreateWithFlags(&c->own, hipStreamNonBlocking) != hipSuccess) {
+    delete c;
+    return ATM_ERR_HIP;
+  }
+  c->stream = c->own;
+  int lds = 0;
+  if (hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, device) == hipSuccess && lds > 0)
+    c->max_lds = (size_t)lds;
+  *out = c;
+  return ATM_OK;
+}
+
+int atm_ctx_destroy(atm_ctx* ctx) {
+  if (!ctx) return ATM_ERR_ARG;
+  hipSetDevice(ctx->device);
+  hipStreamSynchronize(ctx->stream);
+  for (hipEvent_t ev : ctx->events)
+    if (ev) hipEventDestroy(ev);
+  if (ctx->own) hipStreamDestroy(ctx->own);
+  delete ctx;
+  return ATM_OK;
+}
+
+int atm_ctx_set_stream(atm_ctx* ctx, void* hip_stream) {
+  if (!ctx) return ATM_ERR_ARG;
+  ctx->stream = hip_stream ? static_cast<hipStream_t>(hip_stream) : ctx->own;
+  return ATM_OK;
+}
+
+void* atm_ctx_get_stream(atm_ctx* ctx) { return ctx ? (void*)ctx->stream : nullptr; }
+
+const char* atm_last_error(atm_ctx* ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+
+int atm_sync(atm_ctx* ctx) {
+  if (!ctx) return ATM_ERR_ARG;
+  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  return ATM_OK;
+}
+
+int atm_alloc(atm_ctx* ctx, size_t bytes, void** dptr) {
+  if (!ctx || !dptr) return ATM_ERR_ARG;
+  hipSetDevice(ctx->device);
+  hipError_t e = hipMalloc(dptr, bytes ? bytes : 8);
+  if (e == hipErrorOutOfMemory) return fail(ctx, ATM_ERR_NOMEM, "atm_alloc: out of device memory");
+  if (e != hipSuccess) return hipfail(ctx, e, "hipMalloc");
+  return ATM_OK;
+}
+
+int atm_free(atm_ctx* ctx, void* dptr) {
+  if (!ctx) return ATM_ERR_ARG;
+  if (dptr) HIPCHK(ctx, hipFree(dptr));
+  return ATM_OK;
+}
+
+int atm_upload(atm_ctx* ctx, void* dst, const void* src, size_t bytes) {
+  if (!ctx) return ATM_ERR_ARG;
+  HIPCHK(ctx, hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, ctx->stream));
+  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  return ATM_OK;
+}
+
+int atm_download(atm_ctx* ctx, void* dst, const void* src, size_t bytes) {
+  if (!ctx) return ATM_ERR_ARG;
+  HIPCHK(ctx, hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, ctx->stream));
+  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  return ATM_OK;
+}
+
+int atm_copy(atm_ctx* ctx, void* dst, const void* src, size_t bytes) {
+  if (!ctx) return ATM_ERR_ARG;
+  HIPCHK(ctx, hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, ctx->stream));
+  return ATM_OK;
+}
+
+int atm_dg_create(atm_ctx* ctx, const atm_dg_desc* d, atm_dg** out) {
+  if (!ctx || !d || !out) return fail(ctx, ATM_ERR_ARG, "atm_dg_create: null argument");
+  *out = nullptr;
+  if (d->dim < 1 || d->dim > 3) return fail(ctx, ATM_ERR_ARG, "dim must be 1, 2 or 3");
+  if (d->N < 1 || d->N > 7) return fail(ctx, ATM_ERR_UNSUPPORTED, "polynomial degree must be 1..7");
+  if (d->K < 0) return fail(ctx, ATM_ERR_ARG, "K must be >= 0");
+  if (d->dirmask <= 0 || d->dirmask >= (1 << d->dim))
+    return fail(ctx, ATM_ERR_ARG, "directions must be a non-empty subset of 1:dim");
+  if (d->volume_form != ATM_FORM_WEAK && d->volume_form != ATM_FORM_FLUX_DIFFERENCING)
+    return fail(ctx, ATM_ERR_ARG, "unknown volume form");
+  const int Nq = d->N + 1;
+  int Np = 1;
+  for (int i = 0; i < d->dim; ++i) Np *= Nq;
+  Resolved r = resolve(d, Nq, Np);
+  if (!r.fn) return fail(ctx, ATM_ERR_UNSUPPORTED, "balance law does not define this volume flux");
+  if (!sflux_ok(d->law, d->dim, d->surface_flux))
+    return fail(ctx, ATM_ERR_UNSUPPORTED, "balance law does not define this surface flux");
+  if ((d->law == ATM_LAW_EULER_GRAVITY && d->naux < 1 + d->dim) ||
+      (d->law == ATM_LAW_EULER_GRAVITY && d->pde_level_balance && d->naux < 3 + d->dim) ||
+      (d->law == ATM_LAW_LINEAR_EULER_GRAVITY && d->naux < 3 + d->dim))
+    return fail(ctx, ATM_ERR_ARG, "aux state has too few components for the law");
+  if (d->naux > 3 + d->dim) return fail(ctx, ATM_ERR_ARG, "aux state has too many components");
+  if (d->bc == ATM_BC_PRESCRIBED && (d->n_bc_states < 1 || !d->bc_states))
+    return fail(ctx, ATM_ERR_ARG, "prescribed boundary needs bc_states");
+  if (r.lds > ctx->max_lds) return fail(ctx, ATM_ERR_UNSUPPORTED, "element does not fit in LDS");
+  const int Nfpf = Np / Nq, Nfp = 2 * d->dim * Nfpf;
+  const int64_t Ntot = (int64_t)Np * d->K, NF = (int64_t)Nfp * d->K;
+  // Face points must follow the canonical order: face 2r+side, other node
+  // indices ascending with the lower dimension fastest.
+  if (d->K > 0 && !d->faceix_m) return fail(ctx, ATM_ERR_ARG, "missing faceix_m");
+  for (int64_t e = 0; e < d->K; ++e)
+    for (int face = 0; face < 2 * d->dim; ++face) {
+      const int rr = face / 2, side = face % 2;
+      for (int fp = 0; fp < Nfpf; ++fp) {
+        int id[3] = {0, 0, 0}, rem = fp;
+        for (int dd = 0; dd < d->dim; ++dd) {
+          if (dd == rr) continue;
+          id[dd] = rem % Nq;
+          rem /= Nq;
+        }
+        id[rr] = side ? Nq - 1 : 0;
+        const int64_t want = id[0] + Nq * (id[1] + (int64_t)Nq * id[2]) + (int64_t)Np * e;
+        if (d->faceix_m[e * Nfp + face * Nfpf + fp] != want)
+          return fail(ctx, ATM_ERR_UNSUPPORTED, "faceix_m is not in canonical face-point order");
+      }
+    }
+  // every index the kernels follow must stay inside the node array / state table
+  if (d->K > 0 && (!d->faceix_p || !d->bctag || !d->MJ || !d->MJI || !d->g || !d->D || !d->faceMJ || !d->normal))
+    return fail(ctx, ATM_ERR_ARG, "missing geometry array");
+  if (d->naux > 0 && !d->aux) return fail(ctx, ATM_ERR_ARG, "missing aux array");
+  for (int64_t i = 0; i < NF; ++i)
+    if (d->faceix_p[i] < 0 || d->faceix_p[i] >= Ntot) return fail(ctx, ATM_ERR_ARG, "faceix_p out of range");
+  for (int64_t i = 0; i < (int64_t)2 * d->dim * d->K; ++i)
+    if (d->bctag[i] < 0) return fail(ctx, ATM_ERR_ARG, "negative boundary tag");
+  hipSetDevice(ctx->device);
+  atm_dg* dg = new atm_dg();
+  dg->ctx = ctx;
+  dg->law = d->law;
+  dg->dim = d->dim;
+  dg->Nq = Nq;
+  dg->Np = Np;
+  dg->Ns = r.ns;
+  dg->Ntot = Ntot;
+  dg->fn = r.fn;
+  DevDG& p = dg->dev;
+  p.c.gamma = d->gamma;
+  p.c.grav = d->grav;
+  p.c.omega = d->omega;
+  for (int i = 0; i < 3; ++i) p.c.u[i] = d->u[i];
+  p.c.pde_level_balance = d->pde_level_balance;
+  p.c.bc = d->bc;
+  p.c.source = d->source;
+  p.c.n_bc_states = d->n_bc_states;
+  p.K = d->K;
+  p.Nq = Nq;
+  p.Np = Np;
+  p.Nfpf = Nfpf;
+  p.Nfp = Nfp;
+  p.naux = d->naux;
+  p.dirmask = d->dirmask;
+  p.vflux = d->volume_flux;
+  p.sflux = d->surface_flux;
+  p.Ntot = Ntot;
+  p.NF = NF;
+  int rc = ATM_OK;
+  const size_t dd = (size_t)d->dim * d->dim;
+  if ((rc = upload_copy(dg, d->D, (size_t)Nq * Nq, &p.D)) ||
+      (rc = upload_copy(dg, d->g, dd * Ntot, &p.g)) ||
+      (rc = upload_copy(dg, d->MJ, (size_t)Ntot, &p.MJ)) ||
+      (rc = upload_copy(dg, d->MJI, (size_t)Ntot, &p.MJI)) ||
+      (rc = upload_copy(dg, d->faceMJ, (size_t)NF, &p.faceMJ)) ||
+      (rc = upload_copy(dg, d->normal, (size_t)d->dim * NF, &p.normal)) ||
+      (rc = upload_copy(dg, d->aux, (size_t)d->naux * Ntot, &p.aux)) ||
+      (rc = upload_copy(dg, d->faceix_p, (size_t)NF, &p.faceix_p)) ||
+      (rc = upload_copy(dg, d->bctag, (size_t)2 * d->dim * d->K, &p.bctag)) ||
+      (rc = upload_copy(dg, d->bc_states, (size_t)(d->bc == ATM_BC_PRESCRIBED ? d->n_bc_states * r.ns : 0),
+                        &p.c.bc_states))) {
+    atm_dg_destroy(dg);
+    return rc;
+  }
+  *out = dg;
+  return ATM_OK;
+}
+
+int atm_dg_destroy(atm_dg* dg) {
+  if (!dg) return ATM_ERR_ARG;
+  hipSetDevice(dg->ctx->device);
+  hipStreamSynchronize(dg->ctx->stream);
+  for (void* p : dg->allocs) hipFree(p);
+  delete dg;
+  return ATM_OK;
+}
+
+int atm_dg_nstates(const atm_dg* dg) { return dg ? dg->Ns : ATM_ERR_ARG; }
+int64_t atm_dg_nnodes(const atm_dg* dg) { return dg ? dg->Ntot : ATM_ERR_ARG; }
+
+int atm_dg_tendency(atm_dg* dg, double* dq, const double* q, double time, int increment) {
+  (void)time;
+  if (!dg || !dq || !q) return fail(dg ? dg->ctx : nullptr, ATM_ERR_ARG, "atm_dg_tendency: null argument");
+  hipSetDevice(dg->ctx->device);
+  return launch(dg, q, dq, 1.0, increment ? 0 : 1, nullptr, 0.0);
+}
+
+int atm_lsrk_stage(atm_dg* dg, const double* q_in, double* q_out, double* dq, double a, double bdt,
+                   double time) {
+  (void)time;
+  if (!dg || !q_in || !q_out || !dq) return fail(dg ? dg->ctx : nullptr, ATM_ERR_ARG, "atm_lsrk_stage: null argument");
+  if (q_in == q_out) return fail(dg->ctx, ATM_ERR_ARG, "atm_lsrk_stage: q_out must not alias q_in");
+  hipSetDevice(dg->ctx->device);
+  return launch(dg, q_in, dq, a, a == 0.0 ? 1 : 0, q_out, bdt);
+}
+
+int atm_lsrk_run(atm_dg* dg, int scheme, double* q, double* dq, double* scratch, double dt, double t0,
+                 int64_t nsteps, int use_graph) {
+  (void)t0;
+  if (!dg || !q || !dq || !scratch) return fail(dg ? dg->ctx : nullptr, ATM_ERR_ARG, "atm_lsrk_run: null argument");
+  if (scheme != ATM_LSRK54 && scheme != ATM_LSRK144) return fail(dg->ctx, ATM_ERR_ARG, "unknown LSRK scheme");
+  if (q == scratch || dq == scratch || q == dq) return fail(dg->ctx, ATM_ERR_ARG, "atm_lsrk_run: buffers alias");
+  hipSetDevice(dg->ctx->device);
+  const LSRKTable t = lsrk_table(scheme);
+  double* bufs[2] = {q, scratch};
+  int cur = 0;  // buffer holding the current state
+  auto one_step = [&]() -> int {
+    for (int st = 0; st < t.ns; ++st) {
+      int rc = launch(dg, bufs[cur], dq, t.a[st], t.a[st] == 0.0 ? 1 : 0, bufs[1 - cur], t.b[st] * dt);
+      if (rc) return rc;
+      cur = 1 - cur;
+    }
+    return ATM_OK;
+  };
+  // each step flips the buffer parity when the stage count is odd; a pair of
+  // steps always returns the state to q
+  const int64_t pair = (t.ns % 2 == 1) ? 2 : 1;
+  int64_t done = 0;
+  if (use_graph && nsteps >= pair) {
+    hipStream_t s = dg->ctx->stream;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    HIPCHK(dg->ctx, hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+    int rc = ATM_OK;
+    for (int64_t i = 0; i < pair && rc == ATM_OK; ++i) rc = one_step();
+    hipError_t ec = hipStreamEndCapture(s, &graph);
+    if (rc) {
+      if (graph) hipGraphDestroy(graph);
+      return rc;
+    }
+    if (ec != hipSuccess) return hipfail(dg->ctx, ec, "hipStreamEndCapture");
+    HIPCHK(dg->ctx, hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+    const int64_t reps = nsteps / pair;
+    for (int64_t i = 0; i < reps; ++i) {
+      hipError_t e = hipGraphLaunch(exec, s);
+      if (e != hipSuccess) {
+        hipGraphExecDestroy(exec);
+        hipGraphDestroy(graph);
+        return hipfail(dg->ctx, e, "hipGraphLaunch");
+      }
+    }
+    done = reps * pair;
+    hipGraphExecDestroy(exec);
+    hipGraphDestroy(graph);
+  }
+  for (; done < nsteps; ++done) {
+    int rc = one_step();
+    if (rc) return rc;
+  }
+  if (cur != 0) {
+    const int64_t n = (int64_t)dg->Ns * dg->Ntot;
+    hipLaunchKernelGGL(copy_kernel, dim3(ew_blocks(n)), dim3(EW_THREADS), 0, dg->ctx->stream, q, scratch, n);
+    HIPCHK(dg->ctx, hipGetLastError());
+  }
+  return ATM_OK;
+}
+
+int atm_event_record(atm_ctx* ctx, int slot) {
+  if (!ctx || slot < 0 || slot >= 16) return ATM_ERR_ARG;
+  if (!ctx->events[slot]) HIPCHK(ctx, hipEventCreate(&ctx->events[slot]));
+  HIPCHK(ctx, hipEventRecord(ctx->events[slot], ctx->stream));
+  return ATM_OK;
+}
+
+int atm_event_elapsed_ms(atm_ctx* ctx, int a, int b, float* ms) {
+  if (!ctx || a < 0 || a >= 16 || b < 0 || b >= 16 || !ms || !ctx->events[a] || !ctx->events[b]) return ATM_ERR_ARG;
+  HIPCHK(ctx, hipEventSynchronize(ctx->events[b]));
+  HIPCHK(ctx, hipEventElapsedTime(ms, ctx->events[a], ctx->events[b]));
+  return ATM_OK;
+}
+
+int atm_axpby(atm_ctx* ctx, double* y, const double* x, double a, double b, int64_t n) {
+  if (!ctx || !y || !x) return ATM_ERR_ARG;
+  hipLaunchKernelGGL(axpby_kernel, dim3(ew_blocks(n)), dim3(EW_THREADS), 0, ctx->stream, y, x, a, b, n);
+  HIPCHK(ctx, hipGetLastError());
+  return ATM_OK;
+}
+
+// ---- banded -----------------------------------------------------------------
+int atm_banded_create(atm_dg* lin, int32_t KV, const int32_t* colelem, int32_t ncols, double scale,
+                      atm_banded** out) {
+  if (!lin || !colelem || !out || KV < 1 || ncols < 0)
+    return fail(lin ? lin->ctx : nullptr, ATM_ERR_ARG, "atm_banded_create: bad argument");
+  atm_ctx* ctx = lin->ctx;
+  *out = nullptr;
+  if (lin->dev.dirmask != (1 << (lin->dim - 1)))
+    return fail(ctx, ATM_ERR_ARG, "banded operator must act in the last direction only");
+  const int64_t K = lin->dev.K;
+  if ((int64_t)KV * ncols != K) return fail(ctx, ATM_ERR_ARG, "columns must cover every element once");
+  std::vector<int32_t> col(K, -1), ez(K, 0);
+  for (int64_t c = 0; c < ncols; ++c)
+    for (int z = 0; z < KV; ++z) {
+      const int32_t e = colelem[c * KV + z];
+      if (e < 0 || e >= K || col[e] != -1) return fail(ctx, ATM_ERR_ARG, "colelem is not a permutation");
+      col[e] = (int32_t)c;
+      ez[e] = z;
+    }
+  hipSetDevice(ctx->device);
+  atm_banded* b = new atm_banded();
+  b->lin = lin;
+  b->KV = KV;
+  b->ncols = ncols;
+  b->Nh = lin->Np / lin->Nq;
+  b->n = lin->Ns * lin->Nq * KV;
+  b->bw = lin->Ns * lin->Nq - 1;
+  b->nb = 2 * b->bw + 1;
+  b->nlines = (int64_t)ncols * b->Nh;
+  auto cleanup = [&](int rc) {
+    atm_banded_destroy(b);
+    return rc;
+  };
+  const size_t bandsz = (size_t)b->nlines * b->n * b->nb;
+  hipError_t e;
+  if ((e = hipMalloc(&b->band, bandsz * sizeof(double))) != hipSuccess) return cleanup(hipfail(ctx, e, "hipMalloc band"));
+  if ((e = hipMalloc(&b->elem_col, K * sizeof(int32_t) + 8)) != hipSuccess) return cleanup(hipfail(ctx, e, "hipMalloc"));
+  if ((e = hipMalloc(&b->elem_ez, K * sizeof(int32_t) + 8)) != hipSuccess) return cleanup(hipfail(ctx, e, "hipMalloc"));
+  if ((e = hipMalloc(&b->colelem, K * sizeof(int32_t) + 8)) != hipSuccess) return cleanup(hipfail(ctx, e, "hipMalloc"));
+  if (K > 0) {
+    hipMemcpy(b->elem_col, col.data(), K * sizeof(int32_t), hipMemcpyHostToDevice);
+    hipMemcpy(b->elem_ez, ez.data(), K * sizeof(int32_t), hipMemcpyHostToDevice);
+    hipMemcpy(b->colelem, colelem, K * sizeof(int32_t), hipMemcpyHostToDevice);
+  }
+  const int64_t nfield = (int64_t)lin->Ns * lin->Ntot;
+  double *pq = nullptr, *pdq = nullptr;
+  if ((e = hipMalloc(&pq, nfield * sizeof(double) + 8)) != hipSuccess) return cleanup(hipfail(ctx, e, "hipMalloc probe"));
+  if ((e = hipMalloc(&pdq, nfield * sizeof(double) + 8)) != hipSuccess) {
+    hipFree(pq);
+    return cleanup(hipfail(ctx, e, "hipMalloc probe"));
+  }
+  const BandGeo g = band_geo(b);
+  hipStream_t s = ctx->stream;
+  hipLaunchKernelGGL(fill_kernel, dim3(ew_blocks(bandsz)), dim3(EW_THREADS), 0, s, b->band, 0.0, (int64_t)bandsz);
+  int rc = ATM_OK;
+  for (int color = 0; color < b->nb && rc == ATM_OK; ++color) {
+    hipLaunchKernelGGL(band_probe_kernel, dim3(ew_blocks(nfield)), dim3(EW_THREADS), 0, s, g, pq, color);
+    rc = launch(lin, pq, pdq, 1.0, 1, nullptr, 0.0);
+    if (rc == ATM_OK)
+      hipLaunchKernelGGL(band_scatter_kernel, dim3(ew_blocks(b->nlines * b->n)), dim3(EW_THREADS), 0, s, g,
+                         pdq, b->band, color);
+  }
+  if (rc == ATM_OK) {
+    const int64_t blocks = (b->nlines + 63) / 64;
+    if (blocks > 0) hipLaunchKernelGGL(band_lu_kernel, dim3(blocks), dim3(64), 0, s, g, b->band, scale);
+    e = hipStreamSynchronize(s);
+    if (e != hipSuccess) rc = hipfail(ctx, e, "banded assembly");
+  }
+  hipFree(pq);
+  hipFree(pdq);
+  if (rc) return cleanup(rc);
+  *out = b;
+  return ATM_OK;
+}
+
+int atm_banded_destroy(atm_banded* b) {
+  if (!b) return ATM_ERR_ARG;
+  if (b->band) hipFree(b->band);
+  if (b->elem_col) hipFree(b->elem_col);
+  if (b->elem_ez) hipFree(b->elem_ez);
+  if (b->colelem) hipFree(b->colelem);
+  delete b;
+  return ATM_OK;
+}
+
+int atm_banded_solve(atm_banded* b, double* x, const double* rhs) {
+  if (!b || !x || !rhs) return ATM_ERR_ARG;
+  atm_ctx* ctx = b->lin->ctx;
+  const BandGeo g = band_geo(b);
+  const int64_t blocks = (b->nlines + 63) / 64;
+  if (blocks > 0) hipLaunchKernelGGL(band_solve_kernel, dim3(blocks), dim3(64), 0, ctx->stream, g, b->band, x, rhs);
+  HIPCHK(ctx, hipGetLastError());
+  return ATM_OK;
+}
+
+int atm_banded_shape(const atm_banded* b, int64_t* nlines, int32_t* n, int32_t* bw) {
+  if (!b) return ATM_ERR_ARG;
+  if (nlines) *nlines = b->nlines;
+  if (n) *n = b->n;
+  if (bw) *bw = b->bw;
+  return ATM_OK;
+}
+
+// host layout [line][row][bw+col-row]
+int atm_banded_download(atm_banded* b, double* host) {
+  if (!b || !host) return ATM_ERR_ARG;
+  atm_ctx* ctx = b->lin->ctx;
+  const size_t sz = (size_t)b->nlines * b->n * b->nb;
+  std::vector<double> tmp(sz);
+  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  HIPCHK(ctx, hipMemcpy(tmp.data(), b->band, sz * sizeof(double), hipMemcpyDeviceToHost));
+  for (int64_t line = 0; line < b->nlines; ++line)
+    for (int r = 0; r < b->n; ++r)
+      for (int k = 0; k < b->nb; ++k)
+        host[((size_t)line * b->n + r) * b->nb + k] = tmp[((size_t)r * b->nb + k) * b->nlines + line];
+  return ATM_OK;
+}
+
+// ---- ARK23 ------------------------------------------------------------------
+int atm_ark23_run(atm_dg* ex, atm_dg* lin, atm_banded* band, double* q, double* work, double dt, double t0,
+                  int64_t nsteps, int split_rhs, int paperversion) {
+  (void)t0;
+  if (!lin || !band || !q || !work) return fail(lin ? lin->ctx : nullptr, ATM_ERR_ARG, "atm_ark23_run: null argument");
+  atm_ctx* ctx = lin->ctx;
+  if (ex && (ex->Ntot != lin->Ntot || ex->Ns != lin->Ns)) return fail(ctx, ATM_ERR_ARG, "operators differ in shape");
+  if (band->lin != lin) return fail(ctx, ATM_ERR_ARG, "band was not built from lin");
+  hipSetDevice(ctx->device);
+  const int64_t n = (int64_t)lin->Ns * lin->Ntot;
+  // src/odesolvers.jl:284-304
+  const double s2 = std::sqrt(2.0);
+  const double a32 = paperversion ? (3 + 2 * s2) / 6 : 0.5;
+  const double exa[3][3] = {{0, 0, 0}, {2 - s2, 0, 0}, {1 - a32, a32, 0}};
+  const double exb[3] = {1 / (2 * s2), 1 / (2 * s2), 1 - 1 / s2};
+  const double ima[3][3] = {{0, 0, 0}, {1 - 1 / s2, 1 - 1 / s2, 0}, {1 / (2 * s2), 1 / (2 * s2), 1 - 1 / s2}};
+  double* Qhat = work;
+  double* Qs[3] = {q, work + n, work + 2 * n};
+  double* exK[3] = {work + 3 * n, work + 4 * n, work + 5 * n};
+  double* imK[3] = {work + 6 * n, work + 7 * n, work + 8 * n};
+  hipStream_t s = ctx->stream;
+  const int blocks = ew_blocks(n);
+  for (int64_t step = 0; step < nsteps; ++step) {
+    for (int i = 0; i < 3; ++i) {
+      if (i > 0) {
+        ArkCoef c{};
+        for (int k = 0; k < 3; ++k) {
+          c.ea[k] = exa[i][k];
+          c.ia[k] = ima[i][k];
+        }
+        hipLaunchKernelGGL(ark_combine_kernel, dim3(blocks), dim3(EW_THREADS), 0, s, Qhat, q, dt, i, c, exK[0],
+                           exK[1], exK[2], imK[0], imK[1], imK[2], n);
+        int rc = atm_banded_solve(band, Qs[i], Qhat);
+        if (rc) return rc;
+      }
+      int rc;
+      if (ex) {
+        if ((rc = launch(ex, Qs[i], exK[i], 1.0, 1, nullptr, 0.0))) return rc;
+      } else {
+        hipLaunchKernelGGL(fill_kernel, dim3(blocks), dim3(EW_THREADS), 0, s, exK[i], 0.0, n);
+      }
+      if ((rc = launch(lin, Qs[i], imK[i], 1.0, 1, nullptr, 0.0))) return rc;
+      if (!split_rhs) hipLaunchKernelGGL(sub_kernel, dim3(blocks), dim3(EW_THREADS), 0, s, exK[i], imK[i], n);
+    }
+    // q += dt*(b_i exK_i + b~_i imK_i), i = 1..3 (src/odesolvers.jl:382-384)
+    ArkCoef c{};
+    for (int k = 0; k < 3; ++k) {
+      c.ea[k] = exb[k];
+      c.ia[k] = exb[k];
+    }
+    hipLaunchKernelGGL(ark_combine_kernel, dim3(blocks), dim3(EW_THREADS), 0, s, q, q, dt, 3, c, exK[0], exK[1],
+                       exK[2], imK[0], imK[1], imK[2], n);
+    HIPCHK(ctx, hipGetLastError());
+  }
+  return ATM_OK;
+}
+
+}  // extern "C"

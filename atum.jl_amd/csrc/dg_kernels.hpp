@@ -1,0 +1,386 @@
+// DGSEM element kernels for gfx950: volume term + surface term (+ optional
+// low-storage RK update) fused into one launch, one workgroup per `epw`
+// elements, one thread per volume node.
+//
+// Summation order follows the reference kernels so that, compiled without FMA
+// contraction, results match the reference to round-off of log/sqrt only:
+//   weak form:          src/dgsem.jl:217-306 (volumeterm!)
+//   flux differencing:  src/dgsem.jl:470-561 (esvolumeterm_dir!, one direction
+//                       after the other, each summed over n, then s, then d)
+//   surface:            src/dgsem.jl:683-732 (surfaceterm!, directions in order,
+//                       side 1 then 2; every node sits on at most one face per
+//                       direction, so node-centric evaluation keeps that order)
+// Epilogue: dq = (assign ? 0 : alpha*dq) + volume, then the surface terms are
+// subtracted, then optionally q_out = q + bdt*dq (src/odesolvers.jl:51-53).
+#pragma once
+#include "laws.hpp"
+
+namespace atm {
+
+struct DevDG {
+  LawConst c;
+  int K, Nq, Np, Nfpf, Nfp, naux, dirmask, vflux, sflux;
+  int64_t Ntot, NF;
+  const double* D;  // Nq*Nq column major
+  const double* g;
+  const double* MJ;
+  const double* MJI;
+  const double* faceMJ;
+  const double* normal;
+  const double* aux;
+  const int64_t* faceix_p;
+  const int32_t* bctag;
+};
+
+// aux components the two-point fluxes read from neighbour nodes: Φ, ρ_ref, p_ref
+template <class L>
+constexpr int staged_aux() {
+  return (L::LAW == ATM_LAW_EULER_GRAVITY || L::LAW == ATM_LAW_LINEAR_EULER_GRAVITY) ? 3 : 0;
+}
+
+template <class L>
+ATM_DEV void load_aux(const DevDG& p, int64_t node, double* a) {
+#pragma unroll
+  for (int s = 0; s < L::NA; ++s) a[s] = (s < p.naux) ? p.aux[s * p.Ntot + node] : 0.0;
+}
+
+template <class L, int SF>
+ATM_DEV void dispatch_sflux(const DevDG& p, const double* n, const double* qm, const double* am,
+                            const double* qp, const double* ap, double* nf) {
+  if constexpr (SF >= 0) {
+    surfaceflux<L, SF>(p.c, n, qm, am, qp, ap, nf);
+  } else {
+    switch (p.sflux) {
+      case ATM_FLUX_CENTRAL:
+        if constexpr (L::has_sflux(ATM_FLUX_CENTRAL)) surfaceflux<L, ATM_FLUX_CENTRAL>(p.c, n, qm, am, qp, ap, nf);
+        break;
+      case ATM_FLUX_RUSANOV:
+        if constexpr (L::has_sflux(ATM_FLUX_RUSANOV)) surfaceflux<L, ATM_FLUX_RUSANOV>(p.c, n, qm, am, qp, ap, nf);
+        break;
+      case ATM_FLUX_EC:
+        if constexpr (L::has_sflux(ATM_FLUX_EC) && L::has_vflux(ATM_FLUX_EC))
+          surfaceflux<L, ATM_FLUX_EC>(p.c, n, qm, am, qp, ap, nf);
+        break;
+      case ATM_FLUX_KG:
+        if constexpr (L::has_sflux(ATM_FLUX_KG) && L::has_vflux(ATM_FLUX_KG))
+          surfaceflux<L, ATM_FLUX_KG>(p.c, n, qm, am, qp, ap, nf);
+        break;
+      case ATM_FLUX_ROE:
+        if constexpr (L::has_sflux(ATM_FLUX_ROE) && L::LAW != ATM_LAW_ADVECTION &&
+                      L::LAW != ATM_LAW_LINEAR_EULER_GRAVITY)
+          surfaceflux<L, ATM_FLUX_ROE>(p.c, n, qm, am, qp, ap, nf);
+        break;
+      case ATM_FLUX_MATRIX:
+        if constexpr (L::has_sflux(ATM_FLUX_MATRIX) && (L::LAW == ATM_LAW_EULER || L::LAW == ATM_LAW_EULER_GRAVITY))
+          surfaceflux<L, ATM_FLUX_MATRIX>(p.c, n, qm, am, qp, ap, nf);
+        break;
+      default:
+        break;
+    }
+  }
+}
+
+// Surface contributions of the faces this node lies on, subtracted from acc.
+template <class L, int SF>
+ATM_DEV void surface_node(const DevDG& p, int Nq, int64_t e, const int idx[3], const double* q,
+                          const double* qn, const double* an, double MJIn, double* acc) {
+  constexpr int D = L::D, Ns = L::Ns, NA = L::NA;
+#pragma unroll
+  for (int r = 0; r < D; ++r) {
+    if (!((p.dirmask >> r) & 1)) continue;
+    const int ir = idx[r];
+    const int side = (ir == 0) ? 0 : ((ir == Nq - 1) ? 1 : -1);
+    if (side < 0) continue;
+    const int face = 2 * r + side;
+    int fp = 0;
+    if (D == 2) fp = (r == 0) ? idx[1] : idx[0];
+    if (D == 3) fp = (r == 0) ? idx[1] + Nq * idx[2] : ((r == 1) ? idx[0] + Nq * idx[2] : idx[0] + Nq * idx[1]);
+    const int64_t jj = e * p.Nfp + (int64_t)face * p.Nfpf + fp;
+    double n[D];
+#pragma unroll
+    for (int d = 0; d < D; ++d) n[d] = p.normal[d * p.NF + jj];
+    const double fMJ = p.faceMJ[jj];
+    const int tag = p.bctag[e * (2 * D) + face];
+    double qp[Ns], ap[NA], nf[Ns];
+    if (tag == 0) {
+      const int64_t idp = p.faceix_p[jj];
+#pragma unroll
+      for (int s = 0; s < Ns; ++s) qp[s] = q[s * p.Ntot + idp];
+      load_aux<L>(p, idp, ap);
+    } else {
+      boundarystate<L>(p.c, n, qn, tag, qp);
+#pragma unroll
+      for (int s = 0; s < NA; ++s) ap[s] = an[s];
+    }
+    dispatch_sflux<L, SF>(p, n, qn, an, qp, ap, nf);
+#pragma unroll
+    for (int s = 0; s < Ns; ++s) acc[s] -= fMJ * nf[s] * MJIn;
+  }
+}
+
+template <int D>
+ATM_DEV void node_index(int ijk, int Nq, int idx[3]) {
+  idx[0] = ijk % Nq;
+  idx[1] = (D > 1) ? (ijk / Nq) % Nq : 0;
+  idx[2] = (D > 2) ? ijk / (Nq * Nq) : 0;
+}
+
+template <class L>
+ATM_DEV void epilogue(const DevDG& p, int64_t node, const double* q1, const double* acc,
+                      double* __restrict__ dq, double* __restrict__ qout, double bdt) {
+#pragma unroll
+  for (int s = 0; s < L::Ns; ++s) {
+    dq[s * p.Ntot + node] = acc[s];
+    if (qout) qout[s * p.Ntot + node] = q1[s] + bdt * acc[s];
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Flux-differencing form (:per_dir order), all directions of the mask in one launch.
+template <class L, int NQ, int VF, int SF>
+__global__ void __launch_bounds__(512)
+fd_kernel(const DevDG p, const double* __restrict__ q, double* __restrict__ dq, double alpha,
+          int assign, double* __restrict__ qout, double bdt, int epw) {
+  constexpr int D = L::D, Ns = L::Ns, NA = L::NA, SA = staged_aux<L>();
+  const int Nq = NQ > 0 ? NQ : p.Nq;
+  const int Np = NQ > 0 ? (D == 1 ? NQ : (D == 2 ? NQ * NQ : NQ * NQ * NQ)) : p.Np;
+  extern __shared__ double lds[];
+  double* sD = lds;
+  const int stride = (Ns + SA + 3 * D) * Np;
+  const int tid = threadIdx.x;
+  const int el = tid / Np;
+  const int ijk = tid - el * Np;
+  const int64_t e = (int64_t)blockIdx.x * epw + el;
+  const bool active = el < epw && e < p.K;
+  double* sq = lds + Nq * Nq + el * stride;
+  double* sa = sq + Ns * Np;
+  double* slg = sa + SA * Np;
+  for (int t = tid; t < Nq * Nq; t += blockDim.x) sD[t] = p.D[t];
+  double q1[Ns], a1[NA];
+  const int64_t node = e * Np + ijk;
+  if (active) {
+#pragma unroll
+    for (int s = 0; s < Ns; ++s) {
+      q1[s] = q[s * p.Ntot + node];
+      sq[s * Np + ijk] = q1[s];
+    }
+    load_aux<L>(p, node, a1);
+    if constexpr (SA == 3) {
+      sa[ijk] = a1[0];
+      sa[Np + ijk] = a1[D + 1];
+      sa[2 * Np + ijk] = a1[D + 2];
+    }
+    const double MJn = p.MJ[node];
+#pragma unroll
+    for (int r = 0; r < D; ++r)
+      if ((p.dirmask >> r) & 1)
+#pragma unroll
+        for (int d = 0; d < D; ++d) slg[(r * D + d) * Np + ijk] = MJn * p.g[(int64_t)(r + D * d) * p.Ntot + node];
+  }
+  __syncthreads();
+  if (!active) return;
+  int idx[3];
+  node_index<D>(ijk, Nq, idx);
+  const double MJIn = p.MJI[node];
+  double acc[Ns];
+  bool first = true;
+#pragma unroll
+  for (int r = 0; r < D; ++r) {
+    if (!((p.dirmask >> r) & 1)) continue;
+    double dl[Ns];
+#pragma unroll
+    for (int s = 0; s < Ns; ++s) dl[s] = -0.0;
+    L::sources(p.c, dl, q1, a1, 1 << r, false);
+    const int id = idx[r];
+    const int lstride = (r == 0) ? 1 : ((r == 1) ? Nq : Nq * Nq);
+    const int lbase = ijk - id * lstride;
+    const double* lgi = slg + (r * D) * Np;
+    for (int n = 0; n < Nq; ++n) {
+      const int ild = lbase + n * lstride;
+      double q2[Ns], a2[NA];
+#pragma unroll
+      for (int s = 0; s < Ns; ++s) q2[s] = sq[s * Np + ild];
+#pragma unroll
+      for (int s = 0; s < NA; ++s) a2[s] = 0.0;
+      if constexpr (SA == 3) {
+        a2[0] = sa[ild];
+        a2[D + 1] = sa[Np + ild];
+        a2[D + 2] = sa[2 * Np + ild];
+      }
+      double F[D][Ns];
+      L::template twopoint<VF>(p.c, q1, a1, q2, a2, F);
+      const double Ddn = MJIn * sD[id + Nq * n];
+      const double Dnd = MJIn * sD[n + Nq * id];
+#pragma unroll
+      for (int s = 0; s < Ns; ++s) {
+#pragma unroll
+        for (int d = 0; d < D; ++d) {
+          dl[s] -= Ddn * lgi[d * Np + ijk] * F[d][s];
+          dl[s] += F[d][s] * lgi[d * Np + ild] * Dnd;
+        }
+      }
+    }
+    if (first) {
+#pragma unroll
+      for (int s = 0; s < Ns; ++s) acc[s] = assign ? dl[s] : alpha * dq[s * p.Ntot + node] + dl[s];
+      first = false;
+    } else {
+#pragma unroll
+      for (int s = 0; s < Ns; ++s) acc[s] = acc[s] + dl[s];
+    }
+  }
+  if (first) {
+#pragma unroll
+    for (int s = 0; s < Ns; ++s) acc[s] = assign ? 0.0 : alpha * dq[s * p.Ntot + node];
+  }
+  surface_node<L, SF>(p, Nq, e, idx, q, q1, a1, MJIn, acc);
+  epilogue<L>(p, node, q1, acc, dq, qout, bdt);
+}
+
+// ---------------------------------------------------------------------------
+// Weak form (volumeterm! order).
+template <class L, int NQ, int SF>
+__global__ void __launch_bounds__(512)
+weak_kernel(const DevDG p, const double* __restrict__ q, double* __restrict__ dq, double alpha,
+            int assign, double* __restrict__ qout, double bdt, int epw) {
+  constexpr int D = L::D, Ns = L::Ns, NA = L::NA;
+  const int Nq = NQ > 0 ? NQ : p.Nq;
+  const int Np = NQ > 0 ? (D == 1 ? NQ : (D == 2 ? NQ * NQ : NQ * NQ * NQ)) : p.Np;
+  extern __shared__ double lds[];
+  double* sD = lds;
+  const int stride = D * Ns * Np;
+  const int tid = threadIdx.x;
+  const int el = tid / Np;
+  const int ijk = tid - el * Np;
+  const int64_t e = (int64_t)blockIdx.x * epw + el;
+  const bool active = el < epw && e < p.K;
+  double* slF = lds + Nq * Nq + el * stride;  // [dir slot][s][node]
+  for (int t = tid; t < Nq * Nq; t += blockDim.x) sD[t] = p.D[t];
+  double q1[Ns], a1[NA];
+  const int64_t node = e * Np + ijk;
+  if (active) {
+#pragma unroll
+    for (int s = 0; s < Ns; ++s) q1[s] = q[s * p.Ntot + node];
+    load_aux<L>(p, node, a1);
+    double F[D][Ns];
+    L::flux(p.c, q1, a1, F);
+    const double MJn = p.MJ[node];
+#pragma unroll
+    for (int s = 0; s < Ns; ++s) {
+      int slot = 0;
+#pragma unroll
+      for (int r = 0; r < D; ++r) {
+        if (!((p.dirmask >> r) & 1)) continue;
+        double v = 0;
+#pragma unroll
+        for (int dd = 0; dd < D; ++dd) v += p.g[(int64_t)(r + D * dd) * p.Ntot + node] * F[dd][s];
+        v *= MJn;
+        slF[(slot * Ns + s) * Np + ijk] = v;
+        ++slot;
+      }
+    }
+  }
+  __syncthreads();
+  if (!active) return;
+  int idx[3];
+  node_index<D>(ijk, Nq, idx);
+  double dl[Ns];
+#pragma unroll
+  for (int s = 0; s < Ns; ++s) dl[s] = -0.0;
+  L::sources(p.c, dl, q1, a1, p.dirmask, true);
+  const double MJIn = p.MJI[node];
+  const int i = idx[0], j = idx[1], k = idx[2];
+  for (int n = 0; n < Nq; ++n) {
+    const double Dni = sD[n + Nq * i] * MJIn;
+    const double Dnj = sD[n + Nq * j] * MJIn;
+    const double Dnk = sD[n + Nq * k] * MJIn;
+#pragma unroll
+    for (int s = 0; s < Ns; ++s) {
+      int slot = 0;
+      if (p.dirmask & 1) {
+        dl[s] += Dni * slF[(slot * Ns + s) * Np + (ijk - i + n)];
+        ++slot;
+      }
+      if (D > 1 && (p.dirmask & 2)) {
+        dl[s] += Dnj * slF[(slot * Ns + s) * Np + (ijk + (n - j) * Nq)];
+        ++slot;
+      }
+      if (D > 2 && (p.dirmask & 4)) {
+        dl[s] += Dnk * slF[(slot * Ns + s) * Np + (ijk + (n - k) * Nq * Nq)];
+      }
+    }
+  }
+  double acc[Ns];
+#pragma unroll
+  for (int s = 0; s < Ns; ++s) acc[s] = assign ? dl[s] : alpha * dq[s * p.Ntot + node] + dl[s];
+  surface_node<L, SF>(p, Nq, e, idx, q, q1, a1, MJIn, acc);
+  epilogue<L>(p, node, q1, acc, dq, qout, bdt);
+}
+
+// ---------------------------------------------------------------------------
+typedef hipError_t (*LaunchFn)(const DevDG& p, hipStream_t st, const double* q, double* dq,
+                               double alpha, int assign, double* qout, double bdt);
+
+inline int elements_per_workgroup(int Np) { return Np >= 256 ? 1 : 256 / Np; }
+
+template <class L, int NQ, int VF, int SF>
+hipError_t launch_fd(const DevDG& p, hipStream_t st, const double* q, double* dq, double alpha,
+                     int assign, double* qout, double bdt) {
+  constexpr int D = L::D, Ns = L::Ns, SA = staged_aux<L>();
+  const int epw = elements_per_workgroup(p.Np);
+  const int threads = ((epw * p.Np + 63) / 64) * 64;
+  const size_t lds = sizeof(double) * ((size_t)p.Nq * p.Nq + (size_t)epw * (Ns + SA + 3 * D) * p.Np);
+  const int blocks = (p.K + epw - 1) / epw;
+  if (blocks == 0) return hipSuccess;
+  hipLaunchKernelGGL((fd_kernel<L, NQ, VF, SF>), dim3(blocks), dim3(threads), lds, st, p, q, dq,
+                     alpha, assign, qout, bdt, epw);
+  return hipGetLastError();
+}
+
+template <class L, int NQ, int SF>
+hipError_t launch_weak(const DevDG& p, hipStream_t st, const double* q, double* dq, double alpha,
+                       int assign, double* qout, double bdt) {
+  constexpr int D = L::D, Ns = L::Ns;
+  const int epw = elements_per_workgroup(p.Np);
+  const int threads = ((epw * p.Np + 63) / 64) * 64;
+  const size_t lds = sizeof(double) * ((size_t)p.Nq * p.Nq + (size_t)epw * D * Ns * p.Np);
+  const int blocks = (p.K + epw - 1) / epw;
+  if (blocks == 0) return hipSuccess;
+  hipLaunchKernelGGL((weak_kernel<L, NQ, SF>), dim3(blocks), dim3(threads), lds, st, p, q, dq,
+                     alpha, assign, qout, bdt, epw);
+  return hipGetLastError();
+}
+
+// LDS bytes a configuration needs (checked against the device limit at create time)
+template <class L>
+size_t lds_bytes(int form, int Nq, int Np) {
+  constexpr int D = L::D, Ns = L::Ns, SA = staged_aux<L>();
+  const int epw = elements_per_workgroup(Np);
+  if (form == ATM_FORM_WEAK) return sizeof(double) * ((size_t)Nq * Nq + (size_t)epw * D * Ns * Np);
+  return sizeof(double) * ((size_t)Nq * Nq + (size_t)epw * (Ns + SA + 3 * D) * Np);
+}
+
+struct Resolved {
+  LaunchFn fn;
+  size_t lds;
+  int ns;
+};
+
+// Generic (runtime Nq, runtime surface flux) kernel for a law/dimension.
+template <class L>
+Resolved resolve_generic(int form, int vflux, int Nq, int Np) {
+  Resolved r{nullptr, lds_bytes<L>(form, Nq, Np), L::Ns};
+  if (form == ATM_FORM_WEAK) {
+    r.fn = launch_weak<L, 0, -1>;
+  } else {
+    if constexpr (L::has_vflux(ATM_FLUX_CENTRAL))
+      if (vflux == ATM_FLUX_CENTRAL) r.fn = launch_fd<L, 0, ATM_FLUX_CENTRAL, -1>;
+    if constexpr (L::has_vflux(ATM_FLUX_EC))
+      if (vflux == ATM_FLUX_EC) r.fn = launch_fd<L, 0, ATM_FLUX_EC, -1>;
+    if constexpr (L::has_vflux(ATM_FLUX_KG))
+      if (vflux == ATM_FLUX_KG) r.fn = launch_fd<L, 0, ATM_FLUX_KG, -1>;
+  }
+  return r;
+}
+
+}  // namespace atm

@@ -1,0 +1,44 @@
+// Kernel selection: one translation unit per balance law instantiates the
+// element kernels; each exports a resolver that maps an operator description
+// to a launcher.  Hot configurations get kernels with the LGL order and the
+// surface flux fixed at compile time; everything else uses the generic
+// kernels (runtime Nq, runtime surface-flux switch).
+#pragma once
+#include "dg_kernels.hpp"
+
+namespace atm {
+
+Resolved resolve_advection(int dim, int form, int vflux, int sflux, int Nq, int Np);
+Resolved resolve_euler(int dim, int form, int vflux, int sflux, int Nq, int Np);
+Resolved resolve_euler_gravity(int dim, int form, int vflux, int sflux, int Nq, int Np);
+Resolved resolve_linear_euler_gravity(int dim, int form, int vflux, int sflux, int Nq, int Np);
+Resolved resolve_shallow_water(int dim, int form, int vflux, int sflux, int Nq, int Np);
+
+template <template <int> class LAW>
+Resolved resolve_dims(int dim, int form, int vflux, int Nq, int Np) {
+  if (dim == 1) return resolve_generic<LAW<1>>(form, vflux, Nq, Np);
+  if (dim == 2) return resolve_generic<LAW<2>>(form, vflux, Nq, Np);
+  if (dim == 3) return resolve_generic<LAW<3>>(form, vflux, Nq, Np);
+  return Resolved{nullptr, 0, 0};
+}
+
+template <class L>
+bool sflux_supported(int sf) {
+  switch (sf) {
+    case ATM_FLUX_CENTRAL:
+    case ATM_FLUX_RUSANOV:
+      return L::has_sflux(sf);
+    case ATM_FLUX_EC:
+      return L::has_sflux(sf) && L::has_vflux(ATM_FLUX_EC);
+    case ATM_FLUX_KG:
+      return L::has_sflux(sf) && L::has_vflux(ATM_FLUX_KG);
+    case ATM_FLUX_ROE:
+      return L::LAW == ATM_LAW_EULER || L::LAW == ATM_LAW_EULER_GRAVITY || L::LAW == ATM_LAW_SHALLOW_WATER;
+    case ATM_FLUX_MATRIX:
+      return L::LAW == ATM_LAW_EULER || L::LAW == ATM_LAW_EULER_GRAVITY;
+    default:
+      return false;
+  }
+}
+
+}  // namespace atm

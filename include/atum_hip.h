@@ -1,0 +1,198 @@
+/* atum_hip.h — C ABI of libatumhip.so, the gfx950 (MI355X) DGSEM tendency and
+ * time-step library that stands in for Atum.jl's DGSEM / LSRK / ARK hot path.
+ *
+ * Plain C: opaque handles, POD descriptors, double / int64 / int32 pointers,
+ * int status (0 = ok, < 0 = error; details from atm_last_error).  No C++
+ * exceptions and no torch types cross this boundary, so Julia can `ccall` it
+ * directly (the binding is shown in INTEGRATION.md).
+ *
+ * Data layout (the reference's data model, Atum/Bennu field arrays):
+ *   - a field with S components is S contiguous blocks of Np*K doubles; block s
+ *     holds component s as a column-major (Np, K) array, node ijk = i + Nq*(j + Nq*k)
+ *     (0-based here; src/dgsem.jl:247 uses the 1-based form);
+ *   - metric g: dim*dim blocks, block (a + dim*b) = d xi_a / d x_b (Bennu metrics .g);
+ *   - face arrays: (Nfp, K) with Nfp = 2*dim*Nq^(dim-1), grouped by face 2r+side
+ *     (side 0: xi_r = -1), src/dgsem.jl:706-707; node indices are 0-based linear
+ *     indices into the (Np, K) node array;
+ *   - boundary tags: (2*dim, K), 0 = interior or periodic.
+ * Field pointers passed to the compute entry points are DEVICE pointers (allocated
+ * with atm_alloc or by any HIP allocator on the context's device); geometry
+ * pointers in atm_dg_desc are HOST pointers that atm_dg_create copies.
+ * All compute entry points are asynchronous on the context's stream; call
+ * atm_sync before reading results on the host.
+ */
+#ifndef ATUM_HIP_H
+#define ATUM_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* status codes */
+#define ATM_OK 0
+#define ATM_ERR_ARG (-1)
+#define ATM_ERR_UNSUPPORTED (-2)
+#define ATM_ERR_HIP (-3)
+#define ATM_ERR_NOMEM (-4)
+
+/* balance laws (files under src/balancelaws/) */
+#define ATM_LAW_ADVECTION 0
+#define ATM_LAW_EULER 1
+#define ATM_LAW_EULER_GRAVITY 2
+#define ATM_LAW_LINEAR_EULER_GRAVITY 3
+#define ATM_LAW_SHALLOW_WATER 4
+
+/* numerical fluxes (src/numericalfluxes.jl:31-47) */
+#define ATM_FLUX_CENTRAL 0
+#define ATM_FLUX_RUSANOV 1
+#define ATM_FLUX_EC 2
+#define ATM_FLUX_KG 3
+#define ATM_FLUX_ROE 4
+#define ATM_FLUX_MATRIX 5
+
+/* volume forms (src/dgsem.jl:5-14) */
+#define ATM_FORM_WEAK 0
+#define ATM_FORM_FLUX_DIFFERENCING 1
+
+/* boundarystate plug-ins (enumerated; Julia closures cannot cross a C ABI) */
+#define ATM_BC_COPY 0       /* src/balancelaw.jl:21 default */
+#define ATM_BC_REFLECT 1    /* momentum-reflecting wall, experiment/euler_gravity/risingbubble.jl:10-15 */
+#define ATM_BC_ZERO_INFLOW 2 /* advection inflow q+ = 0, test/advection/imex_wave_2d.jl:19-26 */
+#define ATM_BC_PRESCRIBED 3 /* q+ = bc_states[tag-1], experiment/euler/sod.jl:20-23 */
+
+/* problem source! plug-ins */
+#define ATM_SOURCE_NONE 0
+#define ATM_SOURCE_CORIOLIS 1 /* -2Ω x ρu (3-D), experiment/euler_gravity/baroclinicwave.jl:27-34 */
+
+/* time integrators */
+#define ATM_LSRK54 0  /* src/odesolvers.jl:148-174 */
+#define ATM_LSRK144 1 /* src/odesolvers.jl:176-229 */
+
+typedef struct atm_ctx atm_ctx;
+typedef struct atm_dg atm_dg;
+typedef struct atm_banded atm_banded;
+
+/* Everything a DGSEM operator needs: replaces the DGSEM constructor
+ * `DGSEM(; law, grid, surface_numericalflux, volume_form, directions, auxstate)`
+ * (src/dgsem.jl:38-56).  MJ, MJI and faceMJ are passed as computed there. */
+typedef struct atm_dg_desc {
+  int32_t law;            /* ATM_LAW_* */
+  int32_t dim;            /* 1, 2 or 3 */
+  int32_t N;              /* polynomial degree, Nq = N+1 LGL points per direction (1 <= N <= 7) */
+  int32_t K;              /* number of elements */
+  int32_t volume_form;    /* ATM_FORM_* */
+  int32_t volume_flux;    /* two-point flux for flux differencing (ATM_FLUX_CENTRAL/EC/KG) */
+  int32_t surface_flux;   /* ATM_FLUX_* */
+  int32_t dirmask;        /* bit r set <=> direction r+1 in `directions` */
+  int32_t bc;             /* ATM_BC_* */
+  int32_t source;         /* ATM_SOURCE_* */
+  int32_t pde_level_balance; /* EulerGravityLaw pde_level_balance flag */
+  int32_t naux;           /* aux components per node (0 for laws without aux) */
+  int32_t n_bc_states;    /* rows of bc_states (ATM_BC_PRESCRIBED) */
+  int32_t reserved;
+  double gamma, grav, omega;
+  double u[3];            /* advection velocity */
+  const double* bc_states;  /* n_bc_states x Ns, host */
+  const double* D;          /* Nq x Nq column major: D[r + Nq*c] = l_c'(xi_r) */
+  const double* g;          /* dim*dim x (Np*K) */
+  const double* MJ;         /* Np*K */
+  const double* MJI;        /* Np*K */
+  const double* faceMJ;     /* Nfp*K */
+  const double* normal;     /* dim x (Nfp*K) */
+  const double* aux;        /* naux x (Np*K) */
+  const int64_t* faceix_m;  /* Nfp*K */
+  const int64_t* faceix_p;  /* Nfp*K */
+  const int32_t* bctag;     /* 2*dim*K */
+} atm_dg_desc;
+
+/* ---- context ---------------------------------------------------------------
+ * A context binds one device and one HIP stream (default priority).  Not
+ * thread safe: use one context per host thread. */
+int atm_ctx_create(int device, atm_ctx** out);
+int atm_ctx_destroy(atm_ctx* ctx);
+/* Launch on an existing HIP stream (e.g. the caller's current stream) instead of
+ * the context's own; NULL restores the context's stream. */
+int atm_ctx_set_stream(atm_ctx* ctx, void* hip_stream);
+/* the hipStream_t the context launches on (for the caller's event timing) */
+void* atm_ctx_get_stream(atm_ctx* ctx);
+const char* atm_last_error(atm_ctx* ctx);
+int atm_sync(atm_ctx* ctx);
+const char* atm_version(void);
+
+/* ---- device memory ------------------------------------------------------- */
+int atm_alloc(atm_ctx* ctx, size_t bytes, void** dptr);
+int atm_free(atm_ctx* ctx, void* dptr);
+int atm_upload(atm_ctx* ctx, void* dst_device, const void* src_host, size_t bytes);
+int atm_download(atm_ctx* ctx, void* dst_host, const void* src_device, size_t bytes);
+int atm_copy(atm_ctx* ctx, void* dst_device, const void* src_device, size_t bytes);
+
+/* ---- DGSEM operator ------------------------------------------------------ */
+/* Replaces DGSEM(...) (src/dgsem.jl:38-56).  Copies the geometry to the device. */
+int atm_dg_create(atm_ctx* ctx, const atm_dg_desc* desc, atm_dg** out);
+int atm_dg_destroy(atm_dg* dg);
+/* number of state components Ns of the law (numberofstates, src/balancelaw.jl:14) */
+int atm_dg_nstates(const atm_dg* dg);
+/* total nodes Np*K */
+int64_t atm_dg_nnodes(const atm_dg* dg);
+
+/* The rhs! protocol: replaces `(dg::DGSEM)(dq, q, time; increment)`
+ * (src/dgsem.jl:59-98).  increment != 0: dq += L(q); else dq = L(q).
+ * `time` is unused (autonomous laws), as in the reference. */
+int atm_dg_tendency(atm_dg* dg, double* dq, const double* q, double time, int increment);
+
+/* One fused low-storage RK stage (src/odesolvers.jl:50-53):
+ *   dq = a*dq + L(q_in);  q_out = q_in + bdt*dq.
+ * q_out must not alias q_in (neighbour elements read q_in); q_out may alias
+ * nothing else.  a == 0 ignores the previous dq. */
+int atm_lsrk_stage(atm_dg* dg, const double* q_in, double* q_out, double* dq, double a,
+                   double bdt, double time);
+
+/* Replaces dostep!(q, ::LSRK) for `nsteps` steps of size dt (src/odesolvers.jl:47-57).
+ * q (in/out) and dq (in/out, solver storage) are device fields of Ns*Np*K doubles;
+ * scratch is a device field of the same size used as the ping-pong state.
+ * The whole run is launched as one stream of fused stage kernels; with
+ * use_graph != 0 each step is captured once in a hipGraph and replayed. */
+int atm_lsrk_run(atm_dg* dg, int scheme, double* q, double* dq, double* scratch, double dt,
+                 double t0, int64_t nsteps, int use_graph);
+
+/* ---- banded vertical solve (src/bandedsystem.jl + Bennu banded LU / ldiv!) ---
+ * Assembles A = I + scale*L for the linear vertical operator `lin` (its
+ * dirmask must be the last direction only) by probing it with coloured unit
+ * vectors, as Bennu.batchedbandedmatrix does, then factors A (LU, no pivoting).
+ * One matrix per vertical node line: a line is a fixed horizontal node position
+ * inside one column of KV elements; colelem[c*KV + ez] lists the elements of
+ * column c bottom to top.  Unknowns are ordered m = s + Ns*(k + Nq*ez).
+ * For ARK23, scale = -dt * (1 - 1/sqrt(2)) (src/odesolvers.jl:264-268). */
+int atm_banded_create(atm_dg* lin, int32_t KV, const int32_t* colelem_host, int32_t ncols,
+                      double scale, atm_banded** out);
+int atm_banded_destroy(atm_banded* b);
+/* x = A^{-1} b for every line (ldiv!, src/odesolvers.jl:355); x may alias b. */
+int atm_banded_solve(atm_banded* b, double* x, const double* rhs);
+/* Copy the factored band to the host: lines x n x (2*bw+1) in [line][row][bw+col-row]. */
+int atm_banded_shape(const atm_banded* b, int64_t* nlines, int32_t* n, int32_t* bw);
+int atm_banded_download(atm_banded* b, double* host);
+
+/* ---- ARK23 (src/odesolvers.jl:231-388) -------------------------------------
+ * Replaces ARK23(rhs!, linrhs!, q, dt; split_rhs, paperversion) + solve!.
+ * `ex` may be NULL (no explicit operator); `band` must be created from `lin`
+ * with scale = -dt_fac*(1 - 1/sqrt(2)).  work is a device buffer of
+ * 9*Ns*Np*K doubles.  Runs nsteps steps of size dt. */
+int atm_ark23_run(atm_dg* ex, atm_dg* lin, atm_banded* band, double* q, double* work, double dt,
+                  double t0, int64_t nsteps, int split_rhs, int paperversion);
+
+/* ---- timing: HIP events recorded on the context's stream ------------------
+ * slot in [0, 16); atm_event_elapsed_ms waits for the later event. */
+int atm_event_record(atm_ctx* ctx, int slot);
+int atm_event_elapsed_ms(atm_ctx* ctx, int slot_start, int slot_end, float* ms);
+
+/* ---- elementwise helpers (broadcasts of the reference solvers) ------------ */
+/* y = a*y + b*x over n doubles */
+int atm_axpby(atm_ctx* ctx, double* y, const double* x, double a, double b, int64_t n);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* ATUM_HIP_H */
