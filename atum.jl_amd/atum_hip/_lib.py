@@ -30,6 +30,7 @@ LAWS = {"advection": 0, "euler": 1, "euler_gravity": 2, "linear_euler_gravity": 
 FLUX_IDS = {"central": 0, "rusanov": 1, "ec": 2, "kg": 3, "roe": 4, "matrix": 5}
 BC_IDS = {"copy": 0, "reflect": 1, "zero_inflow": 2, "prescribed": 3}
 SOURCE_IDS = {None: 0, "coriolis": 1}
+PRECISION_IDS = {"reference": 0, "fast": 1}
 
 
 class DgDesc(ctypes.Structure):
@@ -37,7 +38,7 @@ class DgDesc(ctypes.Structure):
                 ("volume_form", c_int32), ("volume_flux", c_int32), ("surface_flux", c_int32),
                 ("dirmask", c_int32), ("bc", c_int32), ("source", c_int32),
                 ("pde_level_balance", c_int32), ("naux", c_int32), ("n_bc_states", c_int32),
-                ("reserved", c_int32), ("gamma", c_double), ("grav", c_double),
+                ("precision", c_int32), ("gamma", c_double), ("grav", c_double),
                 ("omega", c_double), ("u", c_double * 3), ("bc_states", _dp), ("D", _dp),
                 ("g", _dp), ("MJ", _dp), ("MJI", _dp), ("faceMJ", _dp), ("normal", _dp),
                 ("aux", _dp), ("faceix_m", _i64p), ("faceix_p", _i64p), ("bctag", _i32p)]

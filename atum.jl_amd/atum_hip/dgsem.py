@@ -54,7 +54,10 @@ class FluxDifferencingForm:
 
 class DGSEM:
     def __init__(self, *, law, grid, surface_numericalflux, volume_form=None, directions=None,
-                 auxstate=None, ctx=None):
+                 auxstate=None, precision="reference", ctx=None):
+        """precision: "reference" keeps the reference's summation order with every
+        product and sum rounded; "fast" evaluates symmetric flux pairs once with FMA
+        (same operator, round-off-level differences)."""
         self.ctx = ctx or _lib.context()
         self.law = law
         self.grid = grid
@@ -89,6 +92,8 @@ class DGSEM:
         d.source = _lib.SOURCE_IDS[prob.source]
         d.pde_level_balance = int(getattr(law, "pde_level_balance", False))
         d.naux = naux
+        d.precision = _lib.PRECISION_IDS[precision]
+        self.precision = precision
         d.gamma = float(getattr(law, "gamma", 1.4))
         d.grav = float(getattr(law, "grav", 9.81))
         d.omega = float(prob.omega)

@@ -259,6 +259,13 @@ LSRKTable lsrk_table(int scheme) {
 
 Resolved resolve(const atm_dg_desc* d, int Nq, int Np) {
   const int f = d->volume_form, vf = d->volume_flux, sf = d->surface_flux;
+  if (d->precision == ATM_PRECISION_FAST && f == ATM_FORM_FLUX_DIFFERENCING && vf == ATM_FLUX_EC &&
+      d->dim >= 2) {
+    Resolved r{nullptr, 0, 0};
+    if (d->law == ATM_LAW_EULER) r = resolve_fast_euler(d->dim, sf, Nq);
+    if (d->law == ATM_LAW_EULER_GRAVITY) r = resolve_fast_euler_gravity(d->dim, sf, Nq);
+    if (r.fn) return r;
+  }
   switch (d->law) {
     case ATM_LAW_ADVECTION: return resolve_advection(d->dim, f, vf, sf, Nq, Np);
     case ATM_LAW_EULER: return resolve_euler(d->dim, f, vf, sf, Nq, Np);
@@ -405,6 +412,8 @@ int atm_dg_create(atm_ctx* ctx, const atm_dg_desc* d, atm_dg** out) {
     return fail(ctx, ATM_ERR_ARG, "directions must be a non-empty subset of 1:dim");
   if (d->volume_form != ATM_FORM_WEAK && d->volume_form != ATM_FORM_FLUX_DIFFERENCING)
     return fail(ctx, ATM_ERR_ARG, "unknown volume form");
+  if (d->precision != ATM_PRECISION_REFERENCE && d->precision != ATM_PRECISION_FAST)
+    return fail(ctx, ATM_ERR_ARG, "unknown precision mode");
   const int Nq = d->N + 1;
   int Np = 1;
   for (int i = 0; i < d->dim; ++i) Np *= Nq;

@@ -13,7 +13,7 @@
 // Epilogue: dq = (assign ? 0 : alpha*dq) + volume, then the surface terms are
 // subtracted, then optionally q_out = q + bdt*dq (src/odesolvers.jl:51-53).
 #pragma once
-#include "laws.hpp"
+#include "pairflux.hpp"
 
 namespace atm {
 
@@ -136,57 +136,56 @@ ATM_DEV void epilogue(const DevDG& p, int64_t node, const double* q1, const doub
 }
 
 // ---------------------------------------------------------------------------
-// Flux-differencing form (:per_dir order), all directions of the mask in one launch.
+// Flux-differencing form (:per_dir order), all directions of the mask in one
+// launch.  Per-node primitives of the two-point flux live in LDS for the whole
+// element; the metric row (MJ*g_r) of the direction being summed is staged in
+// LDS one direction at a time.
 template <class L, int NQ, int VF, int SF>
 __global__ void __launch_bounds__(512)
 fd_kernel(const DevDG p, const double* __restrict__ q, double* __restrict__ dq, double alpha,
           int assign, double* __restrict__ qout, double bdt, int epw) {
-  constexpr int D = L::D, Ns = L::Ns, NA = L::NA, SA = staged_aux<L>();
+  using PF = PairFlux<L, VF>;
+  constexpr int D = L::D, Ns = L::Ns, NA = L::NA, NP = PF::NP;
   const int Nq = NQ > 0 ? NQ : p.Nq;
   const int Np = NQ > 0 ? (D == 1 ? NQ : (D == 2 ? NQ * NQ : NQ * NQ * NQ)) : p.Np;
   extern __shared__ double lds[];
   double* sD = lds;
-  const int stride = (Ns + SA + 3 * D) * Np;
+  const int stride = (NP + D) * Np;
   const int tid = threadIdx.x;
   const int el = tid / Np;
   const int ijk = tid - el * Np;
   const int64_t e = (int64_t)blockIdx.x * epw + el;
   const bool active = el < epw && e < p.K;
-  double* sq = lds + Nq * Nq + el * stride;
-  double* sa = sq + Ns * Np;
-  double* slg = sa + SA * Np;
+  double* sP = lds + Nq * Nq + el * stride;  // [k][node]
+  double* slg = sP + NP * Np;                // [d][node], current direction
   for (int t = tid; t < Nq * Nq; t += blockDim.x) sD[t] = p.D[t];
-  double q1[Ns], a1[NA];
+  double q1[Ns], a1[NA], P1[NP];
   const int64_t node = e * Np + ijk;
+  double MJn = 0;
   if (active) {
 #pragma unroll
-    for (int s = 0; s < Ns; ++s) {
-      q1[s] = q[s * p.Ntot + node];
-      sq[s * Np + ijk] = q1[s];
-    }
+    for (int s = 0; s < Ns; ++s) q1[s] = q[s * p.Ntot + node];
     load_aux<L>(p, node, a1);
-    if constexpr (SA == 3) {
-      sa[ijk] = a1[0];
-      sa[Np + ijk] = a1[D + 1];
-      sa[2 * Np + ijk] = a1[D + 2];
-    }
-    const double MJn = p.MJ[node];
+    PF::prim(p.c, q1, a1, P1);
 #pragma unroll
-    for (int r = 0; r < D; ++r)
-      if ((p.dirmask >> r) & 1)
-#pragma unroll
-        for (int d = 0; d < D; ++d) slg[(r * D + d) * Np + ijk] = MJn * p.g[(int64_t)(r + D * d) * p.Ntot + node];
+    for (int k = 0; k < NP; ++k) sP[k * Np + ijk] = P1[k];
+    MJn = p.MJ[node];
   }
-  __syncthreads();
-  if (!active) return;
   int idx[3];
   node_index<D>(ijk, Nq, idx);
-  const double MJIn = p.MJI[node];
+  const double MJIn = active ? p.MJI[node] : 0.0;
   double acc[Ns];
   bool first = true;
 #pragma unroll
   for (int r = 0; r < D; ++r) {
     if (!((p.dirmask >> r) & 1)) continue;
+    __syncthreads();  // primitives written / previous direction's metric row consumed
+    if (active) {
+#pragma unroll
+      for (int d = 0; d < D; ++d) slg[d * Np + ijk] = MJn * p.g[(int64_t)(r + D * d) * p.Ntot + node];
+    }
+    __syncthreads();
+    if (!active) continue;
     double dl[Ns];
 #pragma unroll
     for (int s = 0; s < Ns; ++s) dl[s] = -0.0;
@@ -194,29 +193,27 @@ fd_kernel(const DevDG p, const double* __restrict__ q, double* __restrict__ dq, 
     const int id = idx[r];
     const int lstride = (r == 0) ? 1 : ((r == 1) ? Nq : Nq * Nq);
     const int lbase = ijk - id * lstride;
-    const double* lgi = slg + (r * D) * Np;
+    double lgi[D];
+#pragma unroll
+    for (int d = 0; d < D; ++d) lgi[d] = slg[d * Np + ijk];
     for (int n = 0; n < Nq; ++n) {
       const int ild = lbase + n * lstride;
-      double q2[Ns], a2[NA];
+      double P2[NP];
 #pragma unroll
-      for (int s = 0; s < Ns; ++s) q2[s] = sq[s * Np + ild];
-#pragma unroll
-      for (int s = 0; s < NA; ++s) a2[s] = 0.0;
-      if constexpr (SA == 3) {
-        a2[0] = sa[ild];
-        a2[D + 1] = sa[Np + ild];
-        a2[D + 2] = sa[2 * Np + ild];
-      }
+      for (int k = 0; k < NP; ++k) P2[k] = sP[k * Np + ild];
       double F[D][Ns];
-      L::template twopoint<VF>(p.c, q1, a1, q2, a2, F);
+      PF::pair(p.c, P1, P2, F);
       const double Ddn = MJIn * sD[id + Nq * n];
       const double Dnd = MJIn * sD[n + Nq * id];
+      double lgn[D];
+#pragma unroll
+      for (int d = 0; d < D; ++d) lgn[d] = slg[d * Np + ild];
 #pragma unroll
       for (int s = 0; s < Ns; ++s) {
 #pragma unroll
         for (int d = 0; d < D; ++d) {
-          dl[s] -= Ddn * lgi[d * Np + ijk] * F[d][s];
-          dl[s] += F[d][s] * lgi[d * Np + ild] * Dnd;
+          dl[s] -= Ddn * lgi[d] * F[d][s];
+          dl[s] += F[d][s] * lgn[d] * Dnd;
         }
       }
     }
@@ -229,6 +226,7 @@ fd_kernel(const DevDG p, const double* __restrict__ q, double* __restrict__ dq, 
       for (int s = 0; s < Ns; ++s) acc[s] = acc[s] + dl[s];
     }
   }
+  if (!active) return;
   if (first) {
 #pragma unroll
     for (int s = 0; s < Ns; ++s) acc[s] = assign ? 0.0 : alpha * dq[s * p.Ntot + node];
@@ -326,10 +324,11 @@ inline int elements_per_workgroup(int Np) { return Np >= 256 ? 1 : 256 / Np; }
 template <class L, int NQ, int VF, int SF>
 hipError_t launch_fd(const DevDG& p, hipStream_t st, const double* q, double* dq, double alpha,
                      int assign, double* qout, double bdt) {
-  constexpr int D = L::D, Ns = L::Ns, SA = staged_aux<L>();
+  constexpr int D = L::D, NP = PairFlux<L, VF>::NP;
   const int epw = elements_per_workgroup(p.Np);
   const int threads = ((epw * p.Np + 63) / 64) * 64;
-  const size_t lds = sizeof(double) * ((size_t)p.Nq * p.Nq + (size_t)epw * (Ns + SA + 3 * D) * p.Np);
+  if (threads > 512) return hipErrorInvalidConfiguration;
+  const size_t lds = sizeof(double) * ((size_t)p.Nq * p.Nq + (size_t)epw * (NP + D) * p.Np);
   const int blocks = (p.K + epw - 1) / epw;
   if (blocks == 0) return hipSuccess;
   hipLaunchKernelGGL((fd_kernel<L, NQ, VF, SF>), dim3(blocks), dim3(threads), lds, st, p, q, dq,
@@ -343,6 +342,7 @@ hipError_t launch_weak(const DevDG& p, hipStream_t st, const double* q, double* 
   constexpr int D = L::D, Ns = L::Ns;
   const int epw = elements_per_workgroup(p.Np);
   const int threads = ((epw * p.Np + 63) / 64) * 64;
+  if (threads > 512) return hipErrorInvalidConfiguration;
   const size_t lds = sizeof(double) * ((size_t)p.Nq * p.Nq + (size_t)epw * D * Ns * p.Np);
   const int blocks = (p.K + epw - 1) / epw;
   if (blocks == 0) return hipSuccess;
@@ -353,11 +353,13 @@ hipError_t launch_weak(const DevDG& p, hipStream_t st, const double* q, double* 
 
 // LDS bytes a configuration needs (checked against the device limit at create time)
 template <class L>
-size_t lds_bytes(int form, int Nq, int Np) {
-  constexpr int D = L::D, Ns = L::Ns, SA = staged_aux<L>();
+size_t lds_bytes(int form, int vflux, int Nq, int Np) {
+  constexpr int D = L::D, Ns = L::Ns;
   const int epw = elements_per_workgroup(Np);
   if (form == ATM_FORM_WEAK) return sizeof(double) * ((size_t)Nq * Nq + (size_t)epw * D * Ns * Np);
-  return sizeof(double) * ((size_t)Nq * Nq + (size_t)epw * (Ns + SA + 3 * D) * Np);
+  int np = D * Ns;  // central: physical flux per node
+  if (vflux != ATM_FLUX_CENTRAL) np = (L::LAW == ATM_LAW_SHALLOW_WATER) ? 2 * D + 3 : D + 4;
+  return sizeof(double) * ((size_t)Nq * Nq + (size_t)epw * (np + D) * Np);
 }
 
 struct Resolved {
@@ -369,7 +371,7 @@ struct Resolved {
 // Generic (runtime Nq, runtime surface flux) kernel for a law/dimension.
 template <class L>
 Resolved resolve_generic(int form, int vflux, int Nq, int Np) {
-  Resolved r{nullptr, lds_bytes<L>(form, Nq, Np), L::Ns};
+  Resolved r{nullptr, lds_bytes<L>(form, vflux, Nq, Np), L::Ns};
   if (form == ATM_FORM_WEAK) {
     r.fn = launch_weak<L, 0, -1>;
   } else {

@@ -1,0 +1,285 @@
+// Pair-symmetric flux-differencing kernel (ATM_PRECISION_FAST).
+//
+// Same operator as the :per_dir kernels of src/dgsem.jl:470-561,
+//   dq_i += MJI_i * sum_r sum_n sum_d G_in[d] * F#(q_i, q_n)[d, s],
+//   G_in[d] = D[n,i] (MJ g_r)_n[d] - D[i,n] (MJ g_r)_i[d],
+// evaluated in a different order: G_ni = -G_in and G_ii = 0, so every
+// unordered pair {a, b} of a line is evaluated once (its diagonal term is
+// identically zero) and its contribution is given to both nodes.  The
+// entropy-conservative flux is symmetric except for the gravity fluctuation
+// term -alpha_a (Phi_a - Phi_b) I with alpha_a = b_avg rho_log / (2 b_a)
+// (src/balancelaws/euler_gravity.jl:231-233), which is applied per side.
+// The logarithmic means use the series of f/atanh(f) and atanh(f)/f in
+// u = f^2, f = (a-b)/(a+b), for u < 0.05 (12 terms, truncation < 1e-17) and
+// logarithms otherwise, so a pair costs two reciprocals.  Results equal the
+// reference to round-off; the summation order (and hence the last bits)
+// differs from the reference's.
+//
+// Work split: thread = node; in direction r the node at line position a
+// evaluates the pairs {a, a+m mod Nq} for m = 1..M (tournament pairing; for
+// even Nq the pairs at distance Nq/2 are taken by the lower half only) and
+// writes the partner's share to an LDS slot that the partner adds after a
+// barrier, in a fixed order (deterministic).
+#pragma once
+#include "dg_kernels.hpp"
+
+namespace atm {
+
+// f/atanh(f) = sum_k LMG[k] u^k and atanh(f)/f = sum_k u^k/(2k+1), u = f^2
+__device__ __forceinline__ double lm_inv_series(double u) {
+#pragma clang fp contract(fast)
+  const double c[12] = {1.0,
+                        -0.33333333333333331,
+                        -0.088888888888888892,
+                        -0.046560846560846560,
+                        -0.030194003527336862,
+                        -0.021796804019026242,
+                        -0.016787551856334924,
+                        -0.013502765051265932,
+                        -0.011203745637718733,
+                        -0.0095160731945278988,
+                        -0.0082312065673505006,
+                        -0.0072244647373939064};
+  double s = c[11];
+#pragma unroll
+  for (int k = 10; k >= 0; --k) s = s * u + c[k];
+  return s;
+}
+
+__device__ __forceinline__ double lm_series(double u) {
+#pragma clang fp contract(fast)
+  double s = 1.0 / 23;
+#pragma unroll
+  for (int k = 10; k >= 0; --k) s = s * u + 1.0 / (2 * k + 1);
+  return s;
+}
+
+constexpr double LM_U0 = 0.05;
+
+// Per-node primitives: [rho, u (D), b, u.u, phi, 1/b]
+template <int D, bool GRAV>
+struct FastEC {
+  static constexpr int Ns = D + 2, NP = D + 5;
+  ATM_DEV static void prim(const LawConst& c, const double* q, const double* a, double* P) {
+#pragma clang fp contract(fast)
+    const double rho = q[0];
+    const double ir = 1 / rho;
+    double u[D];
+#pragma unroll
+    for (int d = 0; d < D; ++d) u[d] = q[1 + d] * ir;
+    const double phi = GRAV ? a[0] : 0.0;
+    double ke = u[0] * q[1];
+#pragma unroll
+    for (int d = 1; d < D; ++d) ke += u[d] * q[1 + d];
+    double p = (c.gamma - 1) * (q[D + 1] - 0.5 * ke);
+    if (GRAV) p = p - (c.gamma - 1) * rho * phi;
+    const double ib = 2 * p * ir;  // 1/b
+    P[0] = rho;
+#pragma unroll
+    for (int d = 0; d < D; ++d) P[1 + d] = u[d];
+    P[D + 1] = 1 / ib;
+    double uu = u[0] * u[0];
+#pragma unroll
+    for (int d = 1; d < D; ++d) uu += u[d] * u[d];
+    P[D + 2] = uu;
+    P[D + 3] = phi;
+    P[D + 4] = ib;
+  }
+
+  // Symmetric part of F#(a, b) and the two gravity fluctuation scalars.
+  ATM_DEV static void pair(const LawConst& c, double inv_gm1, const double* Pa, const double* Pb,
+                           double F[D][Ns], double& fla, double& flb) {
+#pragma clang fp contract(fast)
+    const double ra = Pa[0], rb = Pb[0], ba = Pa[D + 1], bb = Pb[D + 1];
+    const double sr = ra + rb, sb = ba + bb;
+    const double isr = 1 / sr, isb = 1 / sb;
+    const double fr = (ra - rb) * isr, fb = (ba - bb) * isb;
+    const double ur = fr * fr, ub = fb * fb;
+    double Gr, Fb;
+    if (ur < LM_U0 && ub < LM_U0) {
+      Gr = lm_inv_series(ur);
+      Fb = lm_series(ub);
+    } else {
+      Gr = (ur < LM_U0) ? lm_inv_series(ur) : 2 * fr / log(ra / rb);
+      Fb = (ub < LM_U0) ? lm_series(ub) : log(ba / bb) / (2 * fb);
+    }
+    const double rho_log = 0.5 * sr * Gr;
+    const double pd = 0.5 * sr * isb;              // rho_avg / (2 b_avg)
+    const double inv2bl = Fb * isb * inv_gm1;      // 1 / (2 (gamma-1) b_log)
+    double u_avg[D];
+#pragma unroll
+    for (int d = 0; d < D; ++d) u_avg[d] = 0.5 * (Pa[1 + d] + Pb[1 + d]);
+    double cc = inv2bl - 0.25 * (Pa[D + 2] + Pb[D + 2]);
+    if (GRAV) cc += 0.5 * (Pa[D + 3] + Pb[D + 3]);
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+      const double frd = u_avg[d] * rho_log;
+      F[d][0] = frd;
+      double mv = 0;
+#pragma unroll
+      for (int e = 0; e < D; ++e) {
+        double fu = u_avg[e] * frd;  // u_avg[d]*frho[e] with frho[e] = u_avg[e]*rho_log
+        if (d == e) fu += pd;
+        F[d][1 + e] = fu;
+        mv += fu * u_avg[e];
+      }
+      F[d][D + 1] = cc * frd + mv;
+    }
+    if (GRAV) {
+      const double w = 0.25 * sb * rho_log;  // b_avg rho_log / 2
+      const double dphi = Pa[D + 3] - Pb[D + 3];
+      fla = w * Pa[D + 4] * dphi;
+      flb = -w * Pb[D + 4] * dphi;
+    } else {
+      fla = flb = 0.0;
+    }
+  }
+};
+
+template <class L, int NQ, int SF>
+__global__ void __launch_bounds__(512)
+fd_fast_kernel(const DevDG p, const double* __restrict__ q, double* __restrict__ dq, double alpha,
+               int assign, double* __restrict__ qout, double bdt, int epw) {
+#pragma clang fp contract(fast)
+  constexpr bool GRAV = L::LAW == ATM_LAW_EULER_GRAVITY;
+  using FE = FastEC<L::D, GRAV>;
+  constexpr int D = L::D, Ns = L::Ns, NA = L::NA, NP = FE::NP;
+  constexpr int Nq = NQ;
+  constexpr int Np = D == 1 ? NQ : (D == 2 ? NQ * NQ : NQ * NQ * NQ);
+  constexpr int M = Nq / 2;  // pair slots per node and direction
+  extern __shared__ double lds[];
+  double* sD = lds;
+  constexpr int stride = (NP + D + M * Ns) * Np;
+  const int tid = threadIdx.x;
+  const int el = tid / Np;
+  const int ijk = tid - el * Np;
+  const int64_t e = (int64_t)blockIdx.x * epw + el;
+  const bool active = el < epw && e < p.K;
+  double* sP = lds + Nq * Nq + el * stride;  // [k][node]
+  double* slg = sP + NP * Np;                // [d][node]
+  double* sx = slg + D * Np;                 // [m][s][node]
+  for (int t = tid; t < Nq * Nq; t += blockDim.x) sD[t] = p.D[t];
+  double q1[Ns], a1[NA], P1[NP];
+  const int64_t node = e * Np + ijk;
+  double MJn = 0;
+  if (active) {
+#pragma unroll
+    for (int s = 0; s < Ns; ++s) q1[s] = q[s * p.Ntot + node];
+    load_aux<L>(p, node, a1);
+    FE::prim(p.c, q1, a1, P1);
+#pragma unroll
+    for (int k = 0; k < NP; ++k) sP[k * Np + ijk] = P1[k];
+    MJn = p.MJ[node];
+  }
+  int idx[3];
+  node_index<D>(ijk, Nq, idx);
+  const double inv_gm1 = 1 / (p.c.gamma - 1);
+  double vol[Ns];
+#pragma unroll
+  for (int s = 0; s < Ns; ++s) vol[s] = 0.0;
+  double src[Ns];
+#pragma unroll
+  for (int s = 0; s < Ns; ++s) src[s] = 0.0;
+#pragma unroll
+  for (int r = 0; r < D; ++r) {
+    if (!((p.dirmask >> r) & 1)) continue;
+    __syncthreads();
+    if (active) {
+#pragma unroll
+      for (int d = 0; d < D; ++d) slg[d * Np + ijk] = MJn * p.g[(int64_t)(r + D * d) * p.Ntot + node];
+    }
+    __syncthreads();
+    const int a = idx[r];
+    const int lstride = (r == 0) ? 1 : ((r == 1) ? Nq : Nq * Nq);
+    const int lbase = ijk - a * lstride;
+    if (active) {
+      L::sources(p.c, src, q1, a1, 1 << r, false);
+      double lga[D];
+#pragma unroll
+      for (int d = 0; d < D; ++d) lga[d] = slg[d * Np + ijk];
+#pragma unroll
+      for (int m = 1; m <= M; ++m) {
+        if ((Nq % 2 == 0) && m == M && a >= M) continue;
+        const int b = (a + m) % Nq;
+        const int ib = lbase + b * lstride;
+        double P2[NP];
+#pragma unroll
+        for (int k = 0; k < NP; ++k) P2[k] = sP[k * Np + ib];
+        double F[D][Ns], fla, flb;
+        FE::pair(p.c, inv_gm1, P1, P2, F, fla, flb);
+        const double Dba = sD[b + Nq * a], Dab = sD[a + Nq * b];
+        double G[D];
+#pragma unroll
+        for (int d = 0; d < D; ++d) G[d] = Dba * slg[d * Np + ib] - Dab * lga[d];
+        double gf[Ns];
+#pragma unroll
+        for (int s = 0; s < Ns; ++s) {
+          double t = G[0] * F[0][s];
+#pragma unroll
+          for (int d = 1; d < D; ++d) t += G[d] * F[d][s];
+          gf[s] = t;
+        }
+#pragma unroll
+        for (int s = 0; s < Ns; ++s) vol[s] += gf[s];
+#pragma unroll
+        for (int d = 0; d < D; ++d) vol[1 + d] -= G[d] * fla;
+        double* slot = sx + (m - 1) * Ns * Np + ib;
+#pragma unroll
+        for (int s = 0; s < Ns; ++s) slot[s * Np] = -gf[s];
+#pragma unroll
+        for (int d = 0; d < D; ++d) slot[(1 + d) * Np] += G[d] * flb;
+      }
+    }
+    __syncthreads();
+    if (active) {
+#pragma unroll
+      for (int m = 1; m <= M; ++m) {
+        if ((Nq % 2 == 0) && m == M && a < M) continue;
+        const double* slot = sx + (m - 1) * Ns * Np + ijk;
+#pragma unroll
+        for (int s = 0; s < Ns; ++s) vol[s] += slot[s * Np];
+      }
+    }
+  }
+  if (!active) return;
+  const double MJIn = p.MJI[node];
+  double acc[Ns];
+#pragma unroll
+  for (int s = 0; s < Ns; ++s) {
+    const double v = src[s] + MJIn * vol[s];
+    acc[s] = assign ? v : alpha * dq[s * p.Ntot + node] + v;
+  }
+  surface_node<L, SF>(p, Nq, e, idx, q, q1, a1, MJIn, acc);
+  epilogue<L>(p, node, q1, acc, dq, qout, bdt);
+}
+
+template <class L, int NQ>
+constexpr int fast_epw() {
+  constexpr int Np = L::D == 1 ? NQ : (L::D == 2 ? NQ * NQ : NQ * NQ * NQ);
+  return Np >= 64 ? 1 : 256 / Np;
+}
+
+template <class L, int NQ>
+size_t fast_lds_bytes() {
+  constexpr int D = L::D, Ns = L::Ns, NP = FastEC<L::D, true>::NP;
+  constexpr int Np = D == 1 ? NQ : (D == 2 ? NQ * NQ : NQ * NQ * NQ);
+  return sizeof(double) * ((size_t)NQ * NQ + (size_t)fast_epw<L, NQ>() * (NP + D + (NQ / 2) * Ns) * Np);
+}
+
+template <class L, int NQ, int SF>
+hipError_t launch_fd_fast(const DevDG& p, hipStream_t st, const double* q, double* dq, double alpha,
+                          int assign, double* qout, double bdt) {
+  constexpr int D = L::D;
+  constexpr int Np = D == 1 ? NQ : (D == 2 ? NQ * NQ : NQ * NQ * NQ);
+  constexpr int epw = fast_epw<L, NQ>();
+  constexpr int threads = ((epw * Np + 63) / 64) * 64;
+  static_assert(threads <= 512, "workgroup exceeds the kernel's launch bound");
+  const int blocks = (p.K + epw - 1) / epw;
+  if (blocks == 0) return hipSuccess;
+  const size_t lds = fast_lds_bytes<L, NQ>();
+  hipLaunchKernelGGL((fd_fast_kernel<L, NQ, SF>), dim3(blocks), dim3(threads), lds, st, p, q, dq, alpha,
+                     assign, qout, bdt, epw);
+  return hipGetLastError();
+}
+
+}  // namespace atm

@@ -6,7 +6,7 @@ Resolved resolve_shallow_water(int dim, int form, int vflux, int sflux, int Nq, 
   using SW2 = ShallowWater<2>;
   // shallow-water ensemble: 2-D, weak form, Rusanov surface, N = 3..7
   if (dim == 2 && form == ATM_FORM_WEAK && sflux == ATM_FLUX_RUSANOV) {
-    const size_t lds = lds_bytes<SW2>(form, Nq, Np);
+    const size_t lds = lds_bytes<SW2>(form, vflux, Nq, Np);
     switch (Nq) {
       case 4: return Resolved{launch_weak<SW2, 4, ATM_FLUX_RUSANOV>, lds, SW2::Ns};
       case 5: return Resolved{launch_weak<SW2, 5, ATM_FLUX_RUSANOV>, lds, SW2::Ns};

@@ -4,7 +4,7 @@
 // surface flux fixed at compile time; everything else uses the generic
 // kernels (runtime Nq, runtime surface-flux switch).
 #pragma once
-#include "dg_kernels.hpp"
+#include "fd_fast.hpp"
 
 namespace atm {
 
@@ -13,6 +13,32 @@ Resolved resolve_euler(int dim, int form, int vflux, int sflux, int Nq, int Np);
 Resolved resolve_euler_gravity(int dim, int form, int vflux, int sflux, int Nq, int Np);
 Resolved resolve_linear_euler_gravity(int dim, int form, int vflux, int sflux, int Nq, int Np);
 Resolved resolve_shallow_water(int dim, int form, int vflux, int sflux, int Nq, int Np);
+// pair-symmetric EC flux-differencing kernels (ATM_PRECISION_FAST); fn == nullptr if none
+Resolved resolve_fast_euler(int dim, int sflux, int Nq);
+Resolved resolve_fast_euler_gravity(int dim, int sflux, int Nq);
+
+template <class L, int NQ>
+Resolved fast_entry(int sflux) {
+  Resolved r{nullptr, fast_lds_bytes<L, NQ>(), L::Ns};
+  if (sflux == ATM_FLUX_ROE) r.fn = launch_fd_fast<L, NQ, ATM_FLUX_ROE>;
+  else if (sflux == ATM_FLUX_RUSANOV) r.fn = launch_fd_fast<L, NQ, ATM_FLUX_RUSANOV>;
+  else r.fn = launch_fd_fast<L, NQ, -1>;
+  return r;
+}
+
+template <template <int> class LAW, int D>
+Resolved fast_nq(int sflux, int Nq) {
+  switch (Nq) {
+    case 2: return fast_entry<LAW<D>, 2>(sflux);
+    case 3: return fast_entry<LAW<D>, 3>(sflux);
+    case 4: return fast_entry<LAW<D>, 4>(sflux);
+    case 5: return fast_entry<LAW<D>, 5>(sflux);
+    case 6: return fast_entry<LAW<D>, 6>(sflux);
+    case 7: return fast_entry<LAW<D>, 7>(sflux);
+    case 8: return fast_entry<LAW<D>, 8>(sflux);
+    default: return Resolved{nullptr, 0, 0};
+  }
+}
 
 template <template <int> class LAW>
 Resolved resolve_dims(int dim, int form, int vflux, int Nq, int Np) {

@@ -67,6 +67,16 @@ extern "C" {
 #define ATM_SOURCE_NONE 0
 #define ATM_SOURCE_CORIOLIS 1 /* -2Ω x ρu (3-D), experiment/euler_gravity/baroclinicwave.jl:27-34 */
 
+/* arithmetic order of the volume term:
+ *  REFERENCE: the reference's summation order, every product and sum rounded
+ *             (no FMA): agrees with the reference to round-off of log/sqrt;
+ *  FAST:      flux differencing evaluates each symmetric pair once, log means
+ *             by series, FMA contraction: same operator, different last bits
+ *             (EC flux differencing of the Euler laws; other configurations
+ *             run the REFERENCE kernels). */
+#define ATM_PRECISION_REFERENCE 0
+#define ATM_PRECISION_FAST 1
+
 /* time integrators */
 #define ATM_LSRK54 0  /* src/odesolvers.jl:148-174 */
 #define ATM_LSRK144 1 /* src/odesolvers.jl:176-229 */
@@ -92,7 +102,7 @@ typedef struct atm_dg_desc {
   int32_t pde_level_balance; /* EulerGravityLaw pde_level_balance flag */
   int32_t naux;           /* aux components per node (0 for laws without aux) */
   int32_t n_bc_states;    /* rows of bc_states (ATM_BC_PRESCRIBED) */
-  int32_t reserved;
+  int32_t precision;      /* ATM_PRECISION_* */
   double gamma, grav, omega;
   double u[3];            /* advection velocity */
   const double* bc_states;  /* n_bc_states x Ns, host */

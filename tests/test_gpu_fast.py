@@ -1,0 +1,124 @@
+"""Parity of the pair-symmetric FAST flux-differencing kernels with the oracle.
+
+The fast kernels evaluate the same operator as the reference's :per_dir
+kernels in another order (each symmetric pair once, log means by series, FMA),
+so they agree with the reference to round-off:
+  * generic states: tendency relative max-norm <= 1e-12;
+  * near-balanced states (the rising bubble at rest, whose tendency is a small
+    residual of large pressure-gradient and gravity terms): the difference is
+    bounded by the spread between the reference's own :per_dir and
+    :per_dir_symmetric kernels on the same state (x4), since no other summation
+    order can do better than the reference's own variants;
+  * states after 100 steps: relative max-norm <= 1e-10;
+  * EC volume + EC surface: semi-discrete entropy conservation <v(q), L(q)> = 0.
+"""
+import numpy as np
+import pytest
+
+import problems as P
+from gpu_cases import Case, relmax
+
+pytestmark = pytest.mark.gpu
+
+
+def _fast_case(law, dim, N, sflux, **kw):
+    c = Case(law, dim, N, kw.pop("K", 3), form="ec", sflux=sflux, **kw)
+    import atum_hip as A
+    c.dgf = A.DGSEM(law=c.dg.law, grid=c.grid, volume_form=c.dg.volume_form,
+                    surface_numericalflux=c.dg.surface_numericalflux, directions=c.dg.directions,
+                    auxstate=c.aux, precision="fast")
+    return c
+
+
+def _tend(dg, q, dq0=None):
+    qf = dg.fieldarray(q)
+    dqf = dg.fieldarray(dq0 if dq0 is not None else np.zeros_like(q))
+    dg(dqf, qf, 0.0, increment=dq0 is not None)
+    dg.ctx.sync()
+    return dqf.numpy()
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+@pytest.mark.parametrize("N", [1, 2, 3, 4, 5, 6, 7])
+@pytest.mark.parametrize("sflux", ["roe", "rusanov", "ec", "matrix"])
+def test_fast_euler(dim, N, sflux):
+    if dim == 3 and N > 5 and sflux == "matrix":
+        pytest.skip("covered by the other surface fluxes at this size")
+    c = _fast_case("euler", dim, N, sflux, K=2 if dim == 3 else 3, seed=N)
+    ref = c.odg.tendency(c.q)
+    assert relmax(_tend(c.dgf, c.q), ref) <= 1e-12
+    dq0 = c.rng.standard_normal(c.q.shape)
+    ref2 = c.odg.tendency(c.q, dq0.copy(), increment=True)
+    assert relmax(_tend(c.dgf, c.q, dq0), ref2) <= 1e-12
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+@pytest.mark.parametrize("N", [2, 3, 4, 5])
+@pytest.mark.parametrize("sflux", ["roe", "rusanov"])
+def test_fast_euler_gravity(dim, N, sflux):
+    c = _fast_case("euler_gravity", dim, N, sflux, bc="reflect", periodic=(False,) * dim,
+                   gravity_domain=True, seed=3 + N)
+    ref = c.odg.tendency(c.q)
+    assert relmax(_tend(c.dgf, c.q), ref) <= 1e-12
+
+
+def test_fast_bubble_state_within_reference_spread():
+    """Rising bubble at rest, 4^3 elements, N = 4: the tendency is the buoyancy
+    residual of O(1e3) terms; compare against the reference kernels' own spread."""
+    import atum_hip as A
+    from oracle import oracle as O
+    Kd, N = 4, 4
+    v = P.julia_range(0, 2000, Kd + 1)
+    grid = A.brickgrid(A.LobattoCell(N, 3), (v, v, v), (False,) * 3)
+    law = A.EulerGravityLaw(3, grav=9.81, problem=A.Problem(boundary="reflect"))
+    kw = dict(law=law, grid=grid, volume_form=A.FluxDifferencingForm(A.EntropyConservativeFlux()),
+              surface_numericalflux=A.RoeFlux())
+    fast = A.DGSEM(precision="fast", **kw)
+    exact = A.DGSEM(precision="reference", **kw)
+    q = P.rising_bubble(grid.points)
+    og = O.Grid(N, [v, v, v], [False] * 3)
+    aux = P.flat_aux(og.points, 9.81)
+    okw = dict(volume_form="fd", volume_flux="ec", surface_flux="roe", bc="reflect", aux=aux, grav=9.81)
+    ref = O.DG("euler_gravity", og, fd_kernel="per_dir", **okw).tendency(q)
+    ref_sym = O.DG("euler_gravity", og, fd_kernel="per_dir_symmetric", **okw).tendency(q)
+    spread = np.abs(ref_sym - ref).max()
+    got_exact = _tend(exact, q)
+    got_fast = _tend(fast, q)
+    assert relmax(got_exact, ref) <= 1e-12  # the reference-order kernel keeps the 1e-12 bar here too
+    assert np.abs(got_fast - ref).max() <= max(4 * spread, 1e-12 * np.abs(ref).max())
+    # the buoyancy signal itself (vertical momentum) agrees to 1e-9 relative
+    assert relmax(got_fast[3], ref[3]) <= 1e-9
+
+
+def test_fast_vortex_100_steps_and_conservation():
+    import atum_hip as A
+    from oracle import oracle as O
+    K, N = 8, 4
+    v = P.julia_range(-5, 5, K + 1)
+    grid = A.brickgrid(A.LobattoCell(N, 2), (v, v), (True, True))
+    dg = A.DGSEM(law=A.EulerLaw(2), grid=grid,
+                 volume_form=A.FluxDifferencingForm(A.EntropyConservativeFlux()),
+                 surface_numericalflux=A.RusanovFlux(), precision="fast")
+    q0 = P.isentropic_vortex(grid.points, 0.0, center=(0.0, 0.0), u0=(1.0, 1.0))
+    dt = 0.1 * (10 / K) / N
+    ref = q0.copy()
+    O.DG("euler", O.Grid(N, [v, v], [True, True]), volume_form="fd", volume_flux="ec",
+         surface_flux="rusanov").lsrk_steps(ref, dt, 100)
+    q = dg.fieldarray(q0)
+    A.LSRK54(dg, q, dt).run(q, 100)
+    dg.ctx.sync()
+    got = q.numpy()
+    assert relmax(got, ref) <= 1e-10
+    for s in (0, 3):
+        tot0 = np.sum(dg.MJ * q0[s])
+        assert abs(np.sum(dg.MJ * got[s]) - tot0) <= 1e-12 * np.sum(dg.MJ * np.abs(q0[s]))
+
+
+@pytest.mark.parametrize("law,dim", [("euler", 2), ("euler", 3), ("euler_gravity", 3)])
+def test_fast_entropy_conservation(law, dim):
+    """EC volume + EC surface with the fast kernel: <v(q), L(q)>_MJ = 0 to round-off."""
+    kw = dict(bc="reflect", periodic=(True,) * dim, gravity_domain=True) if law == "euler_gravity" else {}
+    c = _fast_case(law, dim, 3, "ec", seed=5, **kw)
+    dq = _tend(c.dgf, c.q)
+    scale = np.sum(c.grid.MJ * np.abs(dq).sum(0))
+    assert abs(c.odg.entropyproduct(c.q, dq)) <= 1e-13 * scale
