@@ -25,9 +25,26 @@ struct atm_dg {
   atm_ctx* ctx = nullptr;
   DevDG dev{};
   LaunchFn fn = nullptr;
+  BatchLaunchFn bfn = nullptr;
+  size_t lds = 0;
+  int epw = 1;
   int law = 0, dim = 0, Nq = 0, Np = 0, Ns = 0;
   int64_t Ntot = 0;
   std::vector<void*> allocs;
+};
+
+struct atm_batch {
+  atm_ctx* ctx = nullptr;
+  struct Group {
+    BatchLaunchFn bfn;
+    int Np, epw, threads;
+    size_t lds;
+    std::vector<int> members;
+    BatchEntry* dtab = nullptr;
+    int64_t blocks = 0;
+  };
+  std::vector<atm_dg*> dgs;
+  std::vector<Group> groups;
 };
 
 struct atm_banded {
@@ -261,7 +278,7 @@ Resolved resolve(const atm_dg_desc* d, int Nq, int Np) {
   const int f = d->volume_form, vf = d->volume_flux, sf = d->surface_flux;
   if (d->precision == ATM_PRECISION_FAST && f == ATM_FORM_FLUX_DIFFERENCING && vf == ATM_FLUX_EC &&
       d->dim >= 2) {
-    Resolved r{nullptr, 0, 0};
+    Resolved r{nullptr, 0, 0, nullptr, 1};
     if (d->law == ATM_LAW_EULER) r = resolve_fast_euler(d->dim, sf, Nq);
     if (d->law == ATM_LAW_EULER_GRAVITY) r = resolve_fast_euler_gravity(d->dim, sf, Nq);
     if (r.fn) return r;
@@ -272,7 +289,7 @@ Resolved resolve(const atm_dg_desc* d, int Nq, int Np) {
     case ATM_LAW_EULER_GRAVITY: return resolve_euler_gravity(d->dim, f, vf, sf, Nq, Np);
     case ATM_LAW_LINEAR_EULER_GRAVITY: return resolve_linear_euler_gravity(d->dim, f, vf, sf, Nq, Np);
     case ATM_LAW_SHALLOW_WATER: return resolve_shallow_water(d->dim, f, vf, sf, Nq, Np);
-    default: return Resolved{nullptr, 0, 0};
+    default: return Resolved{nullptr, 0, 0, nullptr, 1};
   }
 }
 
@@ -468,6 +485,9 @@ int atm_dg_create(atm_ctx* ctx, const atm_dg_desc* d, atm_dg** out) {
   dg->Ns = r.ns;
   dg->Ntot = Ntot;
   dg->fn = r.fn;
+  dg->bfn = r.bfn;
+  dg->lds = r.lds;
+  dg->epw = r.epw;
   DevDG& p = dg->dev;
   p.c.gamma = d->gamma;
   p.c.grav = d->grav;
@@ -615,6 +635,139 @@ int atm_axpby(atm_ctx* ctx, double* y, const double* x, double a, double b, int6
   if (!ctx || !y || !x) return ATM_ERR_ARG;
   hipLaunchKernelGGL(axpby_kernel, dim3(ew_blocks(n)), dim3(EW_THREADS), 0, ctx->stream, y, x, a, b, n);
   HIPCHK(ctx, hipGetLastError());
+  return ATM_OK;
+}
+
+// ---- batched instances --------------------------------------------------------
+int atm_batch_create(atm_ctx* ctx, int32_t n, atm_dg* const* members, atm_batch** out) {
+  if (!ctx || !out || n < 0 || (n > 0 && !members)) return fail(ctx, ATM_ERR_ARG, "atm_batch_create: bad argument");
+  *out = nullptr;
+  atm_batch* b = new atm_batch();
+  b->ctx = ctx;
+  for (int i = 0; i < n; ++i) {
+    atm_dg* dg = members[i];
+    if (!dg || dg->ctx != ctx || !dg->bfn) {
+      delete b;
+      return fail(ctx, ATM_ERR_ARG, "atm_batch_create: member is null, from another context or not batchable");
+    }
+    b->dgs.push_back(dg);
+    int g = -1;
+    for (size_t k = 0; k < b->groups.size(); ++k)
+      if (b->groups[k].bfn == dg->bfn && b->groups[k].Np == dg->Np && b->groups[k].epw == dg->epw &&
+          b->groups[k].lds == dg->lds)
+        g = (int)k;
+    if (g < 0) {
+      atm_batch::Group G{dg->bfn, dg->Np, dg->epw, ((dg->epw * dg->Np + 63) / 64) * 64, dg->lds, {}, nullptr, 0};
+      b->groups.push_back(G);
+      g = (int)b->groups.size() - 1;
+    }
+    b->groups[g].members.push_back(i);
+  }
+  hipSetDevice(ctx->device);
+  for (auto& G : b->groups) {
+    hipError_t e = hipMalloc(&G.dtab, G.members.size() * sizeof(BatchEntry));
+    if (e != hipSuccess) {
+      atm_batch_destroy(b);
+      return hipfail(ctx, e, "hipMalloc batch table");
+    }
+  }
+  *out = b;
+  return ATM_OK;
+}
+
+int atm_batch_destroy(atm_batch* b) {
+  if (!b) return ATM_ERR_ARG;
+  for (auto& G : b->groups)
+    if (G.dtab) hipFree(G.dtab);
+  delete b;
+  return ATM_OK;
+}
+
+int atm_batch_ngroups(const atm_batch* b) { return b ? (int)b->groups.size() : ATM_ERR_ARG; }
+
+int atm_batch_lsrk_run(atm_batch* b, int scheme, double* const* q, double* const* dq, double* const* scratch,
+                       const double* dt, int64_t nsteps, int use_graph) {
+  if (!b || !q || !dq || !scratch || !dt) return fail(b ? b->ctx : nullptr, ATM_ERR_ARG, "atm_batch_lsrk_run: null argument");
+  if (scheme != ATM_LSRK54 && scheme != ATM_LSRK144) return fail(b->ctx, ATM_ERR_ARG, "unknown LSRK scheme");
+  atm_ctx* ctx = b->ctx;
+  hipSetDevice(ctx->device);
+  const int n = (int)b->dgs.size();
+  for (int i = 0; i < n; ++i) {
+    if (!q[i] || !dq[i] || !scratch[i] || q[i] == scratch[i] || q[i] == dq[i] || dq[i] == scratch[i])
+      return fail(ctx, ATM_ERR_ARG, "atm_batch_lsrk_run: member buffers missing or aliased");
+  }
+  // device tables (flattened element table: first workgroup of each member)
+  for (auto& G : b->groups) {
+    std::vector<BatchEntry> tab(G.members.size());
+    int64_t start = 0;
+    for (size_t k = 0; k < G.members.size(); ++k) {
+      const int i = G.members[k];
+      const atm_dg* dg = b->dgs[i];
+      tab[k].p = dg->dev;
+      tab[k].buf[0] = q[i];
+      tab[k].buf[1] = scratch[i];
+      tab[k].dq = dq[i];
+      tab[k].dt = dt[i];
+      tab[k].block_start = start;
+      start += (dg->dev.K + G.epw - 1) / G.epw;
+    }
+    G.blocks = start;
+    HIPCHK(ctx, hipMemcpyAsync(G.dtab, tab.data(), tab.size() * sizeof(BatchEntry), hipMemcpyHostToDevice, ctx->stream));
+  }
+  const LSRKTable t = lsrk_table(scheme);
+  int parity = 0;
+  auto one_step = [&]() -> int {
+    for (int st = 0; st < t.ns; ++st) {
+      for (auto& G : b->groups) {
+        if (G.blocks == 0) continue;
+        hipError_t e = G.bfn(G.dtab, (int)G.members.size(), G.blocks, G.threads, G.lds, G.epw, ctx->stream, parity,
+                             t.a[st], t.a[st] == 0.0 ? 1 : 0, t.b[st]);
+        if (e != hipSuccess) return hipfail(ctx, e, "batched element kernel launch");
+      }
+      parity = 1 - parity;
+    }
+    return ATM_OK;
+  };
+  const int64_t pair = (t.ns % 2 == 1) ? 2 : 1;
+  int64_t done = 0;
+  if (use_graph && nsteps >= pair) {
+    hipStream_t s = ctx->stream;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    HIPCHK(ctx, hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+    int rc = ATM_OK;
+    for (int64_t i = 0; i < pair && rc == ATM_OK; ++i) rc = one_step();
+    hipError_t ec = hipStreamEndCapture(s, &graph);
+    if (rc) {
+      if (graph) hipGraphDestroy(graph);
+      return rc;
+    }
+    if (ec != hipSuccess) return hipfail(ctx, ec, "hipStreamEndCapture");
+    HIPCHK(ctx, hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+    const int64_t reps = nsteps / pair;
+    for (int64_t i = 0; i < reps; ++i) {
+      hipError_t e = hipGraphLaunch(exec, s);
+      if (e != hipSuccess) {
+        hipGraphExecDestroy(exec);
+        hipGraphDestroy(graph);
+        return hipfail(ctx, e, "hipGraphLaunch");
+      }
+    }
+    done = reps * pair;
+    hipGraphExecDestroy(exec);
+    hipGraphDestroy(graph);
+  }
+  for (; done < nsteps; ++done) {
+    int rc = one_step();
+    if (rc) return rc;
+  }
+  if (parity != 0) {
+    for (int i = 0; i < n; ++i) {
+      const int64_t m = (int64_t)b->dgs[i]->Ns * b->dgs[i]->Ntot;
+      hipLaunchKernelGGL(copy_kernel, dim3(ew_blocks(m)), dim3(EW_THREADS), 0, ctx->stream, q[i], scratch[i], m);
+    }
+    HIPCHK(ctx, hipGetLastError());
+  }
   return ATM_OK;
 }
 

@@ -141,20 +141,18 @@ ATM_DEV void epilogue(const DevDG& p, int64_t node, const double* q1, const doub
 // element; the metric row (MJ*g_r) of the direction being summed is staged in
 // LDS one direction at a time.
 template <class L, int NQ, int VF, int SF>
-__global__ void __launch_bounds__(512)
-fd_kernel(const DevDG p, const double* __restrict__ q, double* __restrict__ dq, double alpha,
+ATM_DEV void fd_body(const DevDG& p, double* lds, int64_t blk, const double* __restrict__ q, double* __restrict__ dq, double alpha,
           int assign, double* __restrict__ qout, double bdt, int epw) {
   using PF = PairFlux<L, VF>;
   constexpr int D = L::D, Ns = L::Ns, NA = L::NA, NP = PF::NP;
   const int Nq = NQ > 0 ? NQ : p.Nq;
   const int Np = NQ > 0 ? (D == 1 ? NQ : (D == 2 ? NQ * NQ : NQ * NQ * NQ)) : p.Np;
-  extern __shared__ double lds[];
   double* sD = lds;
   const int stride = (NP + D) * Np;
   const int tid = threadIdx.x;
   const int el = tid / Np;
   const int ijk = tid - el * Np;
-  const int64_t e = (int64_t)blockIdx.x * epw + el;
+  const int64_t e = blk * epw + el;
   const bool active = el < epw && e < p.K;
   double* sP = lds + Nq * Nq + el * stride;  // [k][node]
   double* slg = sP + NP * Np;                // [d][node], current direction
@@ -238,19 +236,17 @@ fd_kernel(const DevDG p, const double* __restrict__ q, double* __restrict__ dq, 
 // ---------------------------------------------------------------------------
 // Weak form (volumeterm! order).
 template <class L, int NQ, int SF>
-__global__ void __launch_bounds__(512)
-weak_kernel(const DevDG p, const double* __restrict__ q, double* __restrict__ dq, double alpha,
+ATM_DEV void weak_body(const DevDG& p, double* lds, int64_t blk, const double* __restrict__ q, double* __restrict__ dq, double alpha,
             int assign, double* __restrict__ qout, double bdt, int epw) {
   constexpr int D = L::D, Ns = L::Ns, NA = L::NA;
   const int Nq = NQ > 0 ? NQ : p.Nq;
   const int Np = NQ > 0 ? (D == 1 ? NQ : (D == 2 ? NQ * NQ : NQ * NQ * NQ)) : p.Np;
-  extern __shared__ double lds[];
   double* sD = lds;
   const int stride = D * Ns * Np;
   const int tid = threadIdx.x;
   const int el = tid / Np;
   const int ijk = tid - el * Np;
-  const int64_t e = (int64_t)blockIdx.x * epw + el;
+  const int64_t e = blk * epw + el;
   const bool active = el < epw && e < p.K;
   double* slF = lds + Nq * Nq + el * stride;  // [dir slot][s][node]
   for (int t = tid; t < Nq * Nq; t += blockDim.x) sD[t] = p.D[t];
@@ -316,6 +312,72 @@ weak_kernel(const DevDG p, const double* __restrict__ q, double* __restrict__ dq
 }
 
 // ---------------------------------------------------------------------------
+// Batched launches: one launch advances many operators that share a kernel.
+// The device table lists, per instance, its operator, its state buffers and its
+// time step, and the first workgroup of its element range (a flattened element
+// table); a workgroup finds its instance by binary search.
+struct BatchEntry {
+  DevDG p;
+  double* buf[2];  // stage reads buf[parity], writes buf[1 - parity]
+  double* dq;
+  double dt;
+  int64_t block_start;
+};
+
+ATM_DEV int batch_find(const BatchEntry* __restrict__ tab, int n) {
+  int lo = 0, hi = n - 1;
+  const int64_t b = blockIdx.x;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (tab[mid].block_start <= b) lo = mid;
+    else hi = mid - 1;
+  }
+  return lo;
+}
+
+#define ATM_KERNEL_PAIR(NAME, BODY, TPARAMS, TARGS)                                                      \
+  template <TPARAMS>                                                                                      \
+  __global__ void __launch_bounds__(512)                                                                  \
+      NAME(const DevDG p, const double* __restrict__ q, double* __restrict__ dq, double alpha, int assign, \
+           double* __restrict__ qout, double bdt, int epw) {                                              \
+    extern __shared__ double lds[];                                                                       \
+    BODY<TARGS>(p, lds, blockIdx.x, q, dq, alpha, assign, qout, bdt, epw);                                \
+  }                                                                                                       \
+  template <TPARAMS>                                                                                      \
+  __global__ void __launch_bounds__(512)                                                                  \
+      NAME##_batch(const BatchEntry* __restrict__ tab, int n, int parity, double alpha, int assign,      \
+                   double b, int epw) {                                                                   \
+    extern __shared__ double lds[];                                                                       \
+    const int i = batch_find(tab, n);                                                                     \
+    const DevDG p = tab[i].p;                                                                             \
+    BODY<TARGS>(p, lds, (int64_t)blockIdx.x - tab[i].block_start, tab[i].buf[parity], tab[i].dq, alpha,   \
+                assign, tab[i].buf[1 - parity], b * tab[i].dt, epw);                                      \
+  }
+
+#define ATM_COMMA ,
+ATM_KERNEL_PAIR(fd_kernel, fd_body, class L ATM_COMMA int NQ ATM_COMMA int VF ATM_COMMA int SF, L ATM_COMMA NQ ATM_COMMA VF ATM_COMMA SF)
+ATM_KERNEL_PAIR(weak_kernel, weak_body, class L ATM_COMMA int NQ ATM_COMMA int SF, L ATM_COMMA NQ ATM_COMMA SF)
+
+typedef hipError_t (*BatchLaunchFn)(const BatchEntry* tab, int n, int64_t blocks, int threads, size_t lds,
+                                    int epw, hipStream_t st, int parity, double alpha, int assign, double b);
+
+template <class L, int NQ, int VF, int SF>
+hipError_t launch_fd_batch(const BatchEntry* tab, int n, int64_t blocks, int threads, size_t lds, int epw,
+                           hipStream_t st, int parity, double alpha, int assign, double b) {
+  hipLaunchKernelGGL((fd_kernel_batch<L, NQ, VF, SF>), dim3(blocks), dim3(threads), lds, st, tab, n, parity,
+                     alpha, assign, b, epw);
+  return hipGetLastError();
+}
+
+template <class L, int NQ, int SF>
+hipError_t launch_weak_batch(const BatchEntry* tab, int n, int64_t blocks, int threads, size_t lds, int epw,
+                             hipStream_t st, int parity, double alpha, int assign, double b) {
+  hipLaunchKernelGGL((weak_kernel_batch<L, NQ, SF>), dim3(blocks), dim3(threads), lds, st, tab, n, parity,
+                     alpha, assign, b, epw);
+  return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
 typedef hipError_t (*LaunchFn)(const DevDG& p, hipStream_t st, const double* q, double* dq,
                                double alpha, int assign, double* qout, double bdt);
 
@@ -366,21 +428,45 @@ struct Resolved {
   LaunchFn fn;
   size_t lds;
   int ns;
+  BatchLaunchFn bfn;
+  int epw;  // elements per workgroup of the launch
 };
+
+template <class L, int NQ, int VF, int SF>
+Resolved make_fd(int Nq, int Np) {
+  return Resolved{launch_fd<L, NQ, VF, SF>, lds_bytes<L>(ATM_FORM_FLUX_DIFFERENCING, VF, Nq, Np), L::Ns,
+                  launch_fd_batch<L, NQ, VF, SF>, elements_per_workgroup(Np)};
+}
+
+template <class L, int NQ, int SF>
+Resolved make_weak(int Nq, int Np) {
+  return Resolved{launch_weak<L, NQ, SF>, lds_bytes<L>(ATM_FORM_WEAK, 0, Nq, Np), L::Ns,
+                  launch_weak_batch<L, NQ, SF>, elements_per_workgroup(Np)};
+}
 
 // Generic (runtime Nq, runtime surface flux) kernel for a law/dimension.
 template <class L>
 Resolved resolve_generic(int form, int vflux, int Nq, int Np) {
-  Resolved r{nullptr, lds_bytes<L>(form, vflux, Nq, Np), L::Ns};
+  Resolved r{nullptr, lds_bytes<L>(form, vflux, Nq, Np), L::Ns, nullptr, elements_per_workgroup(Np)};
   if (form == ATM_FORM_WEAK) {
     r.fn = launch_weak<L, 0, -1>;
+    r.bfn = launch_weak_batch<L, 0, -1>;
   } else {
     if constexpr (L::has_vflux(ATM_FLUX_CENTRAL))
-      if (vflux == ATM_FLUX_CENTRAL) r.fn = launch_fd<L, 0, ATM_FLUX_CENTRAL, -1>;
+      if (vflux == ATM_FLUX_CENTRAL) {
+        r.fn = launch_fd<L, 0, ATM_FLUX_CENTRAL, -1>;
+        r.bfn = launch_fd_batch<L, 0, ATM_FLUX_CENTRAL, -1>;
+      }
     if constexpr (L::has_vflux(ATM_FLUX_EC))
-      if (vflux == ATM_FLUX_EC) r.fn = launch_fd<L, 0, ATM_FLUX_EC, -1>;
+      if (vflux == ATM_FLUX_EC) {
+        r.fn = launch_fd<L, 0, ATM_FLUX_EC, -1>;
+        r.bfn = launch_fd_batch<L, 0, ATM_FLUX_EC, -1>;
+      }
     if constexpr (L::has_vflux(ATM_FLUX_KG))
-      if (vflux == ATM_FLUX_KG) r.fn = launch_fd<L, 0, ATM_FLUX_KG, -1>;
+      if (vflux == ATM_FLUX_KG) {
+        r.fn = launch_fd<L, 0, ATM_FLUX_KG, -1>;
+        r.bfn = launch_fd_batch<L, 0, ATM_FLUX_KG, -1>;
+      }
   }
   return r;
 }

@@ -137,8 +137,7 @@ struct FastEC {
 };
 
 template <class L, int NQ, int SF>
-__global__ void __launch_bounds__(512)
-fd_fast_kernel(const DevDG p, const double* __restrict__ q, double* __restrict__ dq, double alpha,
+ATM_DEV void fd_fast_body(const DevDG& p, double* lds, int64_t blk, const double* __restrict__ q, double* __restrict__ dq, double alpha,
                int assign, double* __restrict__ qout, double bdt, int epw) {
 #pragma clang fp contract(fast)
   constexpr bool GRAV = L::LAW == ATM_LAW_EULER_GRAVITY;
@@ -147,13 +146,12 @@ fd_fast_kernel(const DevDG p, const double* __restrict__ q, double* __restrict__
   constexpr int Nq = NQ;
   constexpr int Np = D == 1 ? NQ : (D == 2 ? NQ * NQ : NQ * NQ * NQ);
   constexpr int M = Nq / 2;  // pair slots per node and direction
-  extern __shared__ double lds[];
   double* sD = lds;
   constexpr int stride = (NP + D + M * Ns) * Np;
   const int tid = threadIdx.x;
   const int el = tid / Np;
   const int ijk = tid - el * Np;
-  const int64_t e = (int64_t)blockIdx.x * epw + el;
+  const int64_t e = blk * epw + el;
   const bool active = el < epw && e < p.K;
   double* sP = lds + Nq * Nq + el * stride;  // [k][node]
   double* slg = sP + NP * Np;                // [d][node]
@@ -253,6 +251,8 @@ fd_fast_kernel(const DevDG p, const double* __restrict__ q, double* __restrict__
   epilogue<L>(p, node, q1, acc, dq, qout, bdt);
 }
 
+ATM_KERNEL_PAIR(fd_fast_kernel, fd_fast_body, class L ATM_COMMA int NQ ATM_COMMA int SF, L ATM_COMMA NQ ATM_COMMA SF)
+
 template <class L, int NQ>
 constexpr int fast_epw() {
   constexpr int Np = L::D == 1 ? NQ : (L::D == 2 ? NQ * NQ : NQ * NQ * NQ);
@@ -279,6 +279,14 @@ hipError_t launch_fd_fast(const DevDG& p, hipStream_t st, const double* q, doubl
   const size_t lds = fast_lds_bytes<L, NQ>();
   hipLaunchKernelGGL((fd_fast_kernel<L, NQ, SF>), dim3(blocks), dim3(threads), lds, st, p, q, dq, alpha,
                      assign, qout, bdt, epw);
+  return hipGetLastError();
+}
+
+template <class L, int NQ, int SF>
+hipError_t launch_fd_fast_batch(const BatchEntry* tab, int n, int64_t blocks, int threads, size_t lds, int epw,
+                                hipStream_t st, int parity, double alpha, int assign, double b) {
+  hipLaunchKernelGGL((fd_fast_kernel_batch<L, NQ, SF>), dim3(blocks), dim3(threads), lds, st, tab, n, parity,
+                     alpha, assign, b, epw);
   return hipGetLastError();
 }
 

@@ -6,7 +6,7 @@ Resolved resolve_linear_euler_gravity(int dim, int form, int vflux, int sflux, i
   using LE3 = LinearEulerGravity<3>;
   // IMEX implicit (vertical) operator: N = 3, weak form, Rusanov surface
   if (dim == 3 && Nq == 4 && form == ATM_FORM_WEAK && sflux == ATM_FLUX_RUSANOV)
-    return Resolved{launch_weak<LE3, 4, ATM_FLUX_RUSANOV>, lds_bytes<LE3>(form, vflux, Nq, Np), LE3::Ns};
+    return make_weak<LE3, 4, ATM_FLUX_RUSANOV>(Nq, Np);
   return resolve_dims<LinearEulerGravity>(dim, form, vflux, Nq, Np);
 }
 }  // namespace atm

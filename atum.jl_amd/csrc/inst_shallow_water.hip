@@ -6,13 +6,12 @@ Resolved resolve_shallow_water(int dim, int form, int vflux, int sflux, int Nq, 
   using SW2 = ShallowWater<2>;
   // shallow-water ensemble: 2-D, weak form, Rusanov surface, N = 3..7
   if (dim == 2 && form == ATM_FORM_WEAK && sflux == ATM_FLUX_RUSANOV) {
-    const size_t lds = lds_bytes<SW2>(form, vflux, Nq, Np);
     switch (Nq) {
-      case 4: return Resolved{launch_weak<SW2, 4, ATM_FLUX_RUSANOV>, lds, SW2::Ns};
-      case 5: return Resolved{launch_weak<SW2, 5, ATM_FLUX_RUSANOV>, lds, SW2::Ns};
-      case 6: return Resolved{launch_weak<SW2, 6, ATM_FLUX_RUSANOV>, lds, SW2::Ns};
-      case 7: return Resolved{launch_weak<SW2, 7, ATM_FLUX_RUSANOV>, lds, SW2::Ns};
-      case 8: return Resolved{launch_weak<SW2, 8, ATM_FLUX_RUSANOV>, lds, SW2::Ns};
+      case 4: return make_weak<SW2, 4, ATM_FLUX_RUSANOV>(Nq, Np);
+      case 5: return make_weak<SW2, 5, ATM_FLUX_RUSANOV>(Nq, Np);
+      case 6: return make_weak<SW2, 6, ATM_FLUX_RUSANOV>(Nq, Np);
+      case 7: return make_weak<SW2, 7, ATM_FLUX_RUSANOV>(Nq, Np);
+      case 8: return make_weak<SW2, 8, ATM_FLUX_RUSANOV>(Nq, Np);
       default: break;
     }
   }

@@ -19,10 +19,17 @@ Resolved resolve_fast_euler_gravity(int dim, int sflux, int Nq);
 
 template <class L, int NQ>
 Resolved fast_entry(int sflux) {
-  Resolved r{nullptr, fast_lds_bytes<L, NQ>(), L::Ns};
-  if (sflux == ATM_FLUX_ROE) r.fn = launch_fd_fast<L, NQ, ATM_FLUX_ROE>;
-  else if (sflux == ATM_FLUX_RUSANOV) r.fn = launch_fd_fast<L, NQ, ATM_FLUX_RUSANOV>;
-  else r.fn = launch_fd_fast<L, NQ, -1>;
+  Resolved r{nullptr, fast_lds_bytes<L, NQ>(), L::Ns, nullptr, fast_epw<L, NQ>()};
+  if (sflux == ATM_FLUX_ROE) {
+    r.fn = launch_fd_fast<L, NQ, ATM_FLUX_ROE>;
+    r.bfn = launch_fd_fast_batch<L, NQ, ATM_FLUX_ROE>;
+  } else if (sflux == ATM_FLUX_RUSANOV) {
+    r.fn = launch_fd_fast<L, NQ, ATM_FLUX_RUSANOV>;
+    r.bfn = launch_fd_fast_batch<L, NQ, ATM_FLUX_RUSANOV>;
+  } else {
+    r.fn = launch_fd_fast<L, NQ, -1>;
+    r.bfn = launch_fd_fast_batch<L, NQ, -1>;
+  }
   return r;
 }
 
@@ -36,7 +43,7 @@ Resolved fast_nq(int sflux, int Nq) {
     case 6: return fast_entry<LAW<D>, 6>(sflux);
     case 7: return fast_entry<LAW<D>, 7>(sflux);
     case 8: return fast_entry<LAW<D>, 8>(sflux);
-    default: return Resolved{nullptr, 0, 0};
+    default: return Resolved{nullptr, 0, 0, nullptr, 1};
   }
 }
 
@@ -45,7 +52,7 @@ Resolved resolve_dims(int dim, int form, int vflux, int Nq, int Np) {
   if (dim == 1) return resolve_generic<LAW<1>>(form, vflux, Nq, Np);
   if (dim == 2) return resolve_generic<LAW<2>>(form, vflux, Nq, Np);
   if (dim == 3) return resolve_generic<LAW<3>>(form, vflux, Nq, Np);
-  return Resolved{nullptr, 0, 0};
+  return Resolved{nullptr, 0, 0, nullptr, 1};
 }
 
 template <class L>

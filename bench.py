@@ -85,6 +85,153 @@ def cpu_baseline(Kd, N, stages):
                       f"({el:.1f} s), FP64 C++ oracle with OpenMP over elements"}
 
 
+def ensemble_workload(args):
+    """256 shallow-water instances (K ~ U{8..64}^2 elements, N ~ U{3..7}, weak form + Rusanov,
+    periodic unit square, seeded Gaussian bumps), LSRK54 with per-instance dt: batched launches
+    (one per N group and stage, hipGraph) against launching each instance's kernels alone."""
+    import numpy as np
+    import atum_hip as A
+    import problems as P
+
+    params = P.sw_ensemble_params(n=args.instances)
+    dgs, q0s, dts = [], [], []
+    for (K, N, amp, w, c) in params:
+        v = P.julia_range(0.0, 1.0, K + 1)
+        grid = A.brickgrid(A.LobattoCell(N, 2), (v, v), (True, True))
+        dgs.append(A.DGSEM(law=A.ShallowWaterLaw(2, grav=9.81), grid=grid,
+                           surface_numericalflux=A.RusanovFlux()))
+        q0s.append(P.sw_bump(grid.points, amp, w, c))
+        dts.append(P.sw_ensemble_dt(K, N, amp))
+    ctx = dgs[0].ctx
+    dof = sum(q.size for q in q0s)
+    stages = 5 * args.steps
+    qs = [dg.fieldarray(q) for dg, q in zip(dgs, q0s)]
+    ens = A.LSRKEnsemble(dgs, qs, dts)
+    ens.run(2)
+    ctx.sync()
+    ctx.record(0)
+    ens.run(args.steps)
+    ctx.record(1)
+    ms_b = ctx.elapsed_ms(0, 1)
+    batched = [q.numpy() for q in qs]
+    if not all(np.all(np.isfinite(b)) for b in batched):
+        raise SystemExit("bench: non-finite ensemble state")
+    per = {}
+    for graph in (False, True):
+        qs2 = [dg.fieldarray(q) for dg, q in zip(dgs, q0s)]
+        solvers = [A.LSRK54(dg, q, dt, use_graph=graph) for dg, q, dt in zip(dgs, qs2, dts)]
+        for s_, q in zip(solvers, qs2):
+            s_.run(q, 2)
+        ctx.sync()
+        ctx.record(2)
+        for s_, q in zip(solvers, qs2):
+            s_.run(q, args.steps)
+        ctx.record(3)
+        per[graph] = ctx.elapsed_ms(2, 3)
+        ctx.sync()
+        same = all(np.array_equal(b, q.numpy()) for b, q in zip(batched, qs2))
+        if not same:
+            raise SystemExit("bench: batched and per-request states differ")
+    value = dof * stages / (ms_b / 1e3)
+    return {
+        "metric": "aggregate FP64 DOF-updates/s per RK stage (batched ensemble)",
+        "value": value, "unit": "DOF-updates/s", "n_gpus": 1, "steps": args.steps, "warmup": 2,
+        "ms_per_step": ms_b / args.steps, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": f"{len(dgs)} shallow-water instances, K~U{{8..64}}^2, N~U{{3..7}}, "
+                               "weak form + Rusanov, LSRK54, per-instance dt (PCG64 seed 20260101)",
+                   "dof": dof, "kernel_groups": ens.batch.ngroups},
+        "per_request_eager": {"value": dof * stages / (per[False] / 1e3), "ms_per_step": per[False] / args.steps},
+        "per_request_graph": {"value": dof * stages / (per[True] / 1e3), "ms_per_step": per[True] / args.steps},
+        "speedup_vs_per_request_eager": per[False] / ms_b,
+        "speedup_vs_per_request_graph": per[True] / ms_b,
+        "bitwise_equal_to_alone": True,
+    }
+
+
+def imex_workload(args):
+    """32x32x10 hex elements on 20 km x 20 km x 10 km, N = 3, Euler-gravity on an isothermal
+    background + 0.1 K seeded theta noise; ARK23 (split_rhs = false): explicit EC flux
+    differencing + Rusanov, implicit vertical linear Euler-gravity with per-line banded LU."""
+    import numpy as np
+    import atum_hip as A
+    import problems as P
+
+    N = 3
+    vx = P.julia_range(0.0, 20e3, 33)
+    vz = P.julia_range(0.0, 10e3, 11)
+    grid = A.brickgrid(A.LobattoCell(N, 3), (vx, vx, vz), (True, True, False), "stacked")
+    prob = A.Problem(boundary="reflect", referencestate=lambda x: P.isothermal_reference(x))
+    law = A.EulerGravityLaw(3, grav=9.81, problem=prob)
+    ex = A.DGSEM(law=law, grid=grid, volume_form=A.FluxDifferencingForm(A.EntropyConservativeFlux()),
+                 surface_numericalflux=A.RusanovFlux(), precision=args.precision)
+    lin = A.DGSEM(law=A.LinearEulerGravityLaw(law), grid=grid, surface_numericalflux=A.RusanovFlux(),
+                  directions=(3,), auxstate=ex.auxstate)
+    aux = ex.auxstate
+    rho0, p0 = aux[4], aux[5]
+    rng = np.random.Generator(np.random.PCG64(20260102))
+    cp, cv = 1.4 * P.CV_D, P.CV_D
+    R = cp - cv
+    T0 = p0 / (rho0 * R)
+    exner = (p0 / 1e5) ** (R / cp)
+    theta = T0 / exner + 0.1 * rng.standard_normal(grid.Ntot)
+    T = theta * exner
+    rho = p0 / (R * T)
+    z = np.zeros_like(rho)
+    q0 = np.stack([rho, z, z, z, rho * (cv * T + aux[0])])
+    dt = 0.2 * grid.min_node_distance(dims=(0, 1)) / 330.0
+    q = ex.fieldarray(q0)
+    ark = A.ARK23(ex, lin, q, dt, split_rhs=False)
+    ctx = ex.ctx
+    ark.run(q, 2)
+    ctx.sync()
+    ctx.record(0)
+    ark.run(q, args.steps)
+    ctx.record(1)
+    ms = ctx.elapsed_ms(0, 1)
+    if not np.all(np.isfinite(q.numpy())):
+        raise SystemExit("bench: non-finite IMEX state")
+    dof = q0.size
+    return {"metric": "FP64 DOF-updates/s per RK stage (IMEX ARK23)", "value": dof * 3 * args.steps / (ms / 1e3),
+            "unit": "DOF-updates/s", "n_gpus": 1, "steps": args.steps, "warmup": 2,
+            "ms_per_step": ms / args.steps, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+            "dtype": "f64", "data": "synthetic",
+            "config": {"workload": "IMEX Euler-gravity 32x32x10, N=3, ARK23 unsplit, banded vertical solve",
+                       "dof": dof, "dt": dt, "lines": 32 * 32 * 16, "precision": args.precision}}
+
+
+def vortex_workload(args):
+    """2-D isentropic vortex, [-5,5]^2 periodic, 8x8, N = 4, EC + Rusanov, LSRK54, CFL 0.1."""
+    import numpy as np
+    import atum_hip as A
+    import problems as P
+
+    K, N = 8, 4
+    v = P.julia_range(-5, 5, K + 1)
+    grid = A.brickgrid(A.LobattoCell(N, 2), (v, v), (True, True))
+    dg = A.DGSEM(law=A.EulerLaw(2), grid=grid, volume_form=A.FluxDifferencingForm(A.EntropyConservativeFlux()),
+                 surface_numericalflux=A.RusanovFlux(), precision=args.precision)
+    q0 = P.isentropic_vortex(grid.points, 0.0, center=(0.0, 0.0), u0=(1.0, 1.0))
+    dt = 0.1 * (10 / K) / N
+    q = dg.fieldarray(q0)
+    s_ = A.LSRK54(dg, q, dt)
+    s_.run(q, 4)
+    dg.ctx.sync()
+    dg.ctx.record(0)
+    s_.run(q, args.steps)
+    dg.ctx.record(1)
+    ms = dg.ctx.elapsed_ms(0, 1)
+    dof = q0.size
+    if not np.all(np.isfinite(q.numpy())):
+        raise SystemExit("bench: non-finite vortex state")
+    return {"metric": "FP64 DOF-updates/s per RK stage (isentropic vortex)", "value": dof * 5 * args.steps / (ms / 1e3),
+            "unit": "DOF-updates/s", "n_gpus": 1, "steps": args.steps, "warmup": 4,
+            "ms_per_step": ms / args.steps, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+            "dtype": "f64", "data": "synthetic",
+            "config": {"workload": "2-D isentropic vortex 8x8, N=4, EC + Rusanov, LSRK54 (CFL 0.1)",
+                       "dof": dof, "us_per_stage": ms / (5 * args.steps) * 1e3, "precision": args.precision}}
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -100,6 +247,9 @@ def main():
                          "reference: the reference's summation order, no FMA")
     ap.add_argument("--no-reference-order", action="store_true",
                     help="skip the extra timing of the reference-order kernel")
+    ap.add_argument("--workload", choices=["bubble", "ensemble", "imex", "vortex"], default="bubble",
+                    help="bubble (default, the metric's configuration); the others are extra lines")
+    ap.add_argument("--instances", type=int, default=256)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build measures one GPU; the path shards by elements with one face "
@@ -110,6 +260,11 @@ def main():
 
     import numpy as np
     import atum_hip as A
+
+    if args.workload != "bubble":
+        fn = {"ensemble": ensemble_workload, "imex": imex_workload, "vortex": vortex_workload}[args.workload]
+        print(json.dumps(fn(args)))
+        return
 
     Kd, N = args.elements, args.N
 

@@ -193,6 +193,23 @@ int atm_banded_download(atm_banded* b, double* host);
 int atm_ark23_run(atm_dg* ex, atm_dg* lin, atm_banded* band, double* q, double* work, double dt,
                   double t0, int64_t nsteps, int split_rhs, int paperversion);
 
+/* ---- batched instances ------------------------------------------------------
+ * Many operators (instances of uneven size and N) advance together: members that
+ * share a kernel (law, dim, N, forms, fluxes, precision) form a group whose
+ * elements are laid out back to back in a flattened element table (per-member
+ * first workgroup), and each group advances in ONE launch per stage; with
+ * use_graph the two-step stage sequence of all groups is one hipGraph.  No
+ * counterpart in the reference (it runs one DGSEM per solver). */
+typedef struct atm_batch atm_batch;
+int atm_batch_create(atm_ctx* ctx, int32_t n, atm_dg* const* members, atm_batch** out);
+int atm_batch_destroy(atm_batch* b);
+/* launches per stage (number of kernel groups) */
+int atm_batch_ngroups(const atm_batch* b);
+/* nsteps LSRK steps for every member i with its own dt[i]: q[i], dq[i], scratch[i] are
+ * device fields of member i (host arrays of n device pointers). */
+int atm_batch_lsrk_run(atm_batch* b, int scheme, double* const* q, double* const* dq,
+                       double* const* scratch, const double* dt, int64_t nsteps, int use_graph);
+
 /* ---- timing: HIP events recorded on the context's stream ------------------
  * slot in [0, 16); atm_event_elapsed_ms waits for the later event. */
 int atm_event_record(atm_ctx* ctx, int slot);

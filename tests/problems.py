@@ -176,3 +176,32 @@ def rising_bubble(x, gamma=1.4, grav=9.81, center=(1000.0, 1000.0, 350.0), radiu
     rhoe = rho * (CV_D * T + phi)
     z = np.zeros_like(rho)
     return np.stack([rho] + [z] * dim + [rhoe])
+
+
+# --- shallow-water ensemble (256 instances of uneven size and degree) ---------
+def sw_ensemble_params(n=256, seed=20260101, kmin=8, kmax=64, nmin=3, nmax=7):
+    """Per instance: K (elements per side), N, bump amplitude, width and centre,
+    drawn with NumPy PCG64 from `seed`."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    out = []
+    for _ in range(n):
+        K = int(rng.integers(kmin, kmax + 1))
+        N = int(rng.integers(nmin, nmax + 1))
+        A = float(rng.uniform(0.01, 0.1))
+        w = float(rng.uniform(0.05, 0.2))
+        c = (float(rng.uniform(0, 1)), float(rng.uniform(0, 1)))
+        out.append((K, N, A, w, c))
+    return out
+
+
+def sw_bump(x, A, w, c):
+    """h = 1 + A exp(-|x - c|^2 / w^2), at rest, zero tracer."""
+    r2 = (x[0] - c[0]) ** 2 + (x[1] - c[1]) ** 2
+    h = 1 + A * np.exp(-r2 / w ** 2)
+    z = np.zeros_like(h)
+    return np.stack([h, z, z, z])
+
+
+def sw_ensemble_dt(K, N, A, grav=9.81):
+    """dt = (1/8) dx / N / sqrt(g h_max) (experiment/shallow_water/bickleyjet.jl:53-54 rule)."""
+    return 0.125 * (1.0 / K) / N / np.sqrt(grav * (1 + A))
