@@ -19,5 +19,6 @@ from .laws import (AdvectionLaw, CentralFlux, DummyProblem, EntropyConservativeF
 from .measures import weightednorm  # noqa: F401
 from .odesolvers import ARK23, LSRK54, LSRK144, solve_  # noqa: F401
 from .batch import Batch, LSRKEnsemble  # noqa: F401
+from .serve import Request, Server, run_alone  # noqa: F401
 
 __all__ = [n for n in dir() if not n.startswith("_")]

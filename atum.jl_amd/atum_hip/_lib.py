@@ -77,6 +77,7 @@ _sig("atm_banded_destroy", c_int, [c_void_p])
 _sig("atm_banded_solve", c_int, [c_void_p, c_void_p, c_void_p])
 _sig("atm_banded_shape", c_int, [c_void_p, _i64p, _i32p, _i32p])
 _sig("atm_banded_download", c_int, [c_void_p, _dp])
+_sig("atm_banded_nunique", c_int64, [c_void_p])
 _sig("atm_ark23_run", c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_double, c_double,
                               c_int64, c_int, c_int])
 _sig("atm_event_record", c_int, [c_void_p, c_int])

@@ -51,7 +51,10 @@ struct atm_banded {
   atm_dg* lin = nullptr;
   int n = 0, bw = 0, nb = 0, KV = 0, ncols = 0, Nh = 0;
   int64_t nlines = 0;
-  double* band = nullptr;      // [(r*nb + bw + c - r) * nlines + line]
+  int64_t nu = 0;               // distinct line matrices
+  double* fac = nullptr;        // LU factors of the distinct matrices: [(r*nb + bw + c - r) * nu + u]
+  int32_t* line_mat = nullptr;  // matrix of each line
+  std::vector<int32_t> line_mat_host;
   int32_t* elem_col = nullptr;  // column of each element (-1: none)
   int32_t* elem_ez = nullptr;   // position of each element in its column
   int32_t* colelem = nullptr;   // ncols*KV
@@ -175,19 +178,59 @@ __global__ void band_scatter_kernel(BandGeo g, const double* __restrict__ dq, do
   }
 }
 
-// A = I + scale*L, then in-place LU without pivoting (unit lower factor).
-__global__ void band_lu_kernel(BandGeo g, double* __restrict__ band, double scale) {
+// A = I + scale*L over the assembled band (layout [(r*nb + j) * nlines + line]).
+__global__ void band_scale_kernel(BandGeo g, double* __restrict__ band, double scale) {
+  const int64_t tot = (int64_t)g.n * g.nb * g.nlines;
+  for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < tot; t += (int64_t)gridDim.x * blockDim.x) {
+    const int j = (int)((t / g.nlines) % g.nb);
+    double v = band[t] * scale;
+    if (j == g.bw) v += 1;
+    band[t] = v;
+  }
+}
+
+// 64-bit FNV-1a hash of the bit patterns of one line's band.
+__global__ void band_hash_kernel(BandGeo g, const double* __restrict__ band, uint64_t* __restrict__ hash) {
   const int64_t line = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   if (line >= g.nlines) return;
+  uint64_t h = 1469598103934665603ull;
+  const int64_t m = (int64_t)g.n * g.nb;
+  for (int64_t k = 0; k < m; ++k) {
+    h ^= (uint64_t)__double_as_longlong(band[k * g.nlines + line]);
+    h *= 1099511628211ull;
+  }
+  hash[line] = h;
+}
+
+// same[line] = 1 if the line's band equals the band of rep[line] bit for bit.
+__global__ void band_verify_kernel(BandGeo g, const double* __restrict__ band, const int64_t* __restrict__ rep,
+                                   int32_t* __restrict__ same) {
+  const int64_t line = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (line >= g.nlines) return;
+  const int64_t o = rep[line];
+  const int64_t m = (int64_t)g.n * g.nb;
+  int32_t eq = 1;
+  for (int64_t k = 0; k < m && eq; ++k)
+    eq = __double_as_longlong(band[k * g.nlines + line]) == __double_as_longlong(band[k * g.nlines + o]);
+  same[line] = eq;
+}
+
+// fac[k * nu + u] = band[k * nlines + src[u]]
+__global__ void band_compact_kernel(BandGeo g, const double* __restrict__ band, const int64_t* __restrict__ src,
+                                    int64_t nu, double* __restrict__ fac) {
+  const int64_t tot = (int64_t)g.n * g.nb * nu;
+  for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < tot; t += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t k = t / nu, u = t - k * nu;
+    fac[t] = band[k * g.nlines + src[u]];
+  }
+}
+
+// In-place LU without pivoting (unit lower factor), one thread per distinct matrix.
+__global__ void band_lu_kernel(BandGeo g, double* __restrict__ fac, int64_t nu) {
+  const int64_t u = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (u >= nu) return;
   const int n = g.n, bw = g.bw, nb = g.nb;
-  const int64_t L = g.nlines;
-  auto A = [&](int r, int c) -> double& { return band[((int64_t)r * nb + (bw + c - r)) * L + line]; };
-  for (int r = 0; r < n; ++r)
-    for (int c = (r - bw > 0 ? r - bw : 0); c <= (r + bw < n - 1 ? r + bw : n - 1); ++c) {
-      double v = A(r, c) * scale;
-      if (r == c) v += 1;
-      A(r, c) = v;
-    }
+  auto A = [&](int r, int c) -> double& { return fac[((int64_t)r * nb + (bw + c - r)) * nu + u]; };
   for (int k = 0; k < n; ++k) {
     const double piv = A(k, k);
     const int iend = k + bw < n - 1 ? k + bw : n - 1;
@@ -199,27 +242,55 @@ __global__ void band_lu_kernel(BandGeo g, double* __restrict__ band, double scal
   }
 }
 
-// Forward and back substitution, one thread per line; x may alias b.
-__global__ void band_solve_kernel(BandGeo g, const double* __restrict__ band, double* x, const double* b) {
-  const int64_t line = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+// Forward and back substitution, one thread (lane) per vertical line; the last
+// bw unknowns of a line live in an LDS window; unknowns are visited with
+// incremental addressing (element, vertical node, state).  x may alias b.
+constexpr int BAND_WIN = 64;  // > bw for every supported (Ns, Nq)
+
+__global__ void __launch_bounds__(64)
+band_solve_kernel(BandGeo g, const double* __restrict__ fac, const int32_t* __restrict__ line_mat, int64_t nu,
+                  double* x, const double* b) {
+  __shared__ double win[BAND_WIN][64];
+  const int lane = threadIdx.x;
+  const int64_t line = blockIdx.x * (int64_t)blockDim.x + lane;
   if (line >= g.nlines) return;
   const int n = g.n, bw = g.bw, nb = g.nb;
-  const int64_t L = g.nlines;
-  auto A = [&](int r, int c) { return band[((int64_t)r * nb + (bw + c - r)) * L + line]; };
-  auto idx = [&](int r) {
-    int s;
-    const int64_t node = band_node(g, line, r, &s);
-    return (int64_t)s * g.Ntot + node;
-  };
-  for (int r = 0; r < n; ++r) {
-    double v = b[idx(r)];
-    for (int c = (r - bw > 0 ? r - bw : 0); c < r; ++c) v -= A(r, c) * x[idx(c)];
-    x[idx(r)] = v;
+  const int64_t mat = line_mat[line];
+  const int64_t c = line / g.Nh;
+  const int h = (int)(line - c * g.Nh);
+  const int32_t* col = g.colelem + c * g.KV;
+  int r = 0;
+  for (int ez = 0; ez < g.KV; ++ez) {
+    const int64_t base_e = h + (int64_t)g.Np * col[ez];
+    for (int k = 0; k < g.Nq; ++k) {
+      const int64_t base = base_e + (int64_t)g.Nh * k;
+      for (int s = 0; s < g.Ns; ++s, ++r) {
+        const int64_t idx = (int64_t)s * g.Ntot + base;
+        double v = b[idx];
+        const int lo = r - bw > 0 ? r - bw : 0;
+        const double* arow = fac + ((int64_t)r * nb + bw - r) * nu + mat;
+        for (int cc = lo; cc < r; ++cc) v -= arow[(int64_t)cc * nu] * win[cc & (BAND_WIN - 1)][lane];
+        win[r & (BAND_WIN - 1)][lane] = v;
+        x[idx] = v;
+      }
+    }
   }
-  for (int r = n - 1; r >= 0; --r) {
-    double v = x[idx(r)];
-    for (int c = r + 1; c <= (r + bw < n - 1 ? r + bw : n - 1); ++c) v -= A(r, c) * x[idx(c)];
-    x[idx(r)] = v / A(r, r);
+  r = n - 1;
+  for (int ez = g.KV - 1; ez >= 0; --ez) {
+    const int64_t base_e = h + (int64_t)g.Np * col[ez];
+    for (int k = g.Nq - 1; k >= 0; --k) {
+      const int64_t base = base_e + (int64_t)g.Nh * k;
+      for (int s = g.Ns - 1; s >= 0; --s, --r) {
+        const int64_t idx = (int64_t)s * g.Ntot + base;
+        double v = x[idx];
+        const int hi = r + bw < n - 1 ? r + bw : n - 1;
+        const double* arow = fac + ((int64_t)r * nb + bw - r) * nu + mat;
+        for (int cc = r + 1; cc <= hi; ++cc) v -= arow[(int64_t)cc * nu] * win[cc & (BAND_WIN - 1)][lane];
+        v = v / arow[(int64_t)r * nu];
+        win[r & (BAND_WIN - 1)][lane] = v;
+        x[idx] = v;
+      }
+    }
   }
 }
 
@@ -806,7 +877,8 @@ int atm_banded_create(atm_dg* lin, int32_t KV, const int32_t* colelem, int32_t n
   };
   const size_t bandsz = (size_t)b->nlines * b->n * b->nb;
   hipError_t e;
-  if ((e = hipMalloc(&b->band, bandsz * sizeof(double))) != hipSuccess) return cleanup(hipfail(ctx, e, "hipMalloc band"));
+  double* band = nullptr;  // assembled band of every line, [(r*nb + j) * nlines + line]
+  if ((e = hipMalloc(&band, bandsz * sizeof(double) + 8)) != hipSuccess) return cleanup(hipfail(ctx, e, "hipMalloc band"));
   if ((e = hipMalloc(&b->elem_col, K * sizeof(int32_t) + 8)) != hipSuccess) return cleanup(hipfail(ctx, e, "hipMalloc"));
   if ((e = hipMalloc(&b->elem_ez, K * sizeof(int32_t) + 8)) != hipSuccess) return cleanup(hipfail(ctx, e, "hipMalloc"));
   if ((e = hipMalloc(&b->colelem, K * sizeof(int32_t) + 8)) != hipSuccess) return cleanup(hipfail(ctx, e, "hipMalloc"));
@@ -817,38 +889,111 @@ int atm_banded_create(atm_dg* lin, int32_t KV, const int32_t* colelem, int32_t n
   }
   const int64_t nfield = (int64_t)lin->Ns * lin->Ntot;
   double *pq = nullptr, *pdq = nullptr;
-  if ((e = hipMalloc(&pq, nfield * sizeof(double) + 8)) != hipSuccess) return cleanup(hipfail(ctx, e, "hipMalloc probe"));
+  if ((e = hipMalloc(&pq, nfield * sizeof(double) + 8)) != hipSuccess) {
+    hipFree(band);
+    return cleanup(hipfail(ctx, e, "hipMalloc probe"));
+  }
   if ((e = hipMalloc(&pdq, nfield * sizeof(double) + 8)) != hipSuccess) {
     hipFree(pq);
+    hipFree(band);
     return cleanup(hipfail(ctx, e, "hipMalloc probe"));
   }
   const BandGeo g = band_geo(b);
   hipStream_t s = ctx->stream;
-  hipLaunchKernelGGL(fill_kernel, dim3(ew_blocks(bandsz)), dim3(EW_THREADS), 0, s, b->band, 0.0, (int64_t)bandsz);
+  hipLaunchKernelGGL(fill_kernel, dim3(ew_blocks(bandsz)), dim3(EW_THREADS), 0, s, band, 0.0, (int64_t)bandsz);
   int rc = ATM_OK;
   for (int color = 0; color < b->nb && rc == ATM_OK; ++color) {
     hipLaunchKernelGGL(band_probe_kernel, dim3(ew_blocks(nfield)), dim3(EW_THREADS), 0, s, g, pq, color);
     rc = launch(lin, pq, pdq, 1.0, 1, nullptr, 0.0);
     if (rc == ATM_OK)
       hipLaunchKernelGGL(band_scatter_kernel, dim3(ew_blocks(b->nlines * b->n)), dim3(EW_THREADS), 0, s, g,
-                         pdq, b->band, color);
-  }
-  if (rc == ATM_OK) {
-    const int64_t blocks = (b->nlines + 63) / 64;
-    if (blocks > 0) hipLaunchKernelGGL(band_lu_kernel, dim3(blocks), dim3(64), 0, s, g, b->band, scale);
-    e = hipStreamSynchronize(s);
-    if (e != hipSuccess) rc = hipfail(ctx, e, "banded assembly");
+                         pdq, band, color);
   }
   hipFree(pq);
   hipFree(pdq);
-  if (rc) return cleanup(rc);
+  if (rc) {
+    hipFree(band);
+    return cleanup(rc);
+  }
+  // A = I + scale*L, then keep one copy of each distinct line matrix
+  hipLaunchKernelGGL(band_scale_kernel, dim3(ew_blocks(bandsz)), dim3(EW_THREADS), 0, s, g, band, scale);
+  const int64_t L = b->nlines;
+  uint64_t* dhash = nullptr;
+  int64_t* drep = nullptr;
+  int32_t* dsame = nullptr;
+  std::vector<uint64_t> hash(L);
+  std::vector<int64_t> rep(L);
+  std::vector<int32_t> same(L);
+  auto done = [&](int code) {
+    if (dhash) hipFree(dhash);
+    if (drep) hipFree(drep);
+    if (dsame) hipFree(dsame);
+    hipFree(band);
+    return code ? cleanup(code) : ATM_OK;
+  };
+  if ((e = hipMalloc(&dhash, L * sizeof(uint64_t) + 8)) != hipSuccess) return done(hipfail(ctx, e, "hipMalloc"));
+  if ((e = hipMalloc(&drep, L * sizeof(int64_t) + 8)) != hipSuccess) return done(hipfail(ctx, e, "hipMalloc"));
+  if ((e = hipMalloc(&dsame, L * sizeof(int32_t) + 8)) != hipSuccess) return done(hipfail(ctx, e, "hipMalloc"));
+  const int64_t lblocks = (L + 63) / 64;
+  if (lblocks > 0) hipLaunchKernelGGL(band_hash_kernel, dim3(lblocks), dim3(64), 0, s, g, band, dhash);
+  if ((e = hipMemcpyAsync(hash.data(), dhash, L * sizeof(uint64_t), hipMemcpyDeviceToHost, s)) != hipSuccess ||
+      (e = hipStreamSynchronize(s)) != hipSuccess)
+    return done(hipfail(ctx, e, "banded assembly"));
+  {
+    std::vector<std::pair<uint64_t, int64_t>> first;
+    first.reserve(64);
+    for (int64_t l = 0; l < L; ++l) {
+      int64_t r0 = -1;
+      for (auto& pr : first)
+        if (pr.first == hash[l]) { r0 = pr.second; break; }
+      if (r0 < 0) {
+        if (first.size() < 4096) first.push_back({hash[l], l});
+        r0 = l;
+      }
+      rep[l] = r0;
+    }
+  }
+  if ((e = hipMemcpyAsync(drep, rep.data(), L * sizeof(int64_t), hipMemcpyHostToDevice, s)) != hipSuccess)
+    return done(hipfail(ctx, e, "hipMemcpy"));
+  if (lblocks > 0) hipLaunchKernelGGL(band_verify_kernel, dim3(lblocks), dim3(64), 0, s, g, band, drep, dsame);
+  if ((e = hipMemcpyAsync(same.data(), dsame, L * sizeof(int32_t), hipMemcpyDeviceToHost, s)) != hipSuccess ||
+      (e = hipStreamSynchronize(s)) != hipSuccess)
+    return done(hipfail(ctx, e, "banded verification"));
+  std::vector<int64_t> src;
+  std::vector<int64_t> uid(L, -1);
+  b->line_mat_host.assign(L, 0);
+  for (int64_t l = 0; l < L; ++l) {
+    const int64_t r0 = same[l] ? rep[l] : l;
+    if (uid[r0] < 0) {
+      uid[r0] = (int64_t)src.size();
+      src.push_back(r0);
+    }
+    b->line_mat_host[l] = (int32_t)uid[r0];
+  }
+  b->nu = (int64_t)src.size();
+  const size_t facsz = (size_t)b->n * b->nb * b->nu;
+  if ((e = hipMalloc(&b->fac, facsz * sizeof(double) + 8)) != hipSuccess) return done(hipfail(ctx, e, "hipMalloc factors"));
+  if ((e = hipMalloc(&b->line_mat, L * sizeof(int32_t) + 8)) != hipSuccess) return done(hipfail(ctx, e, "hipMalloc"));
+  if ((e = hipMemcpyAsync(drep, src.data(), src.size() * sizeof(int64_t), hipMemcpyHostToDevice, s)) != hipSuccess ||
+      (e = hipMemcpyAsync(b->line_mat, b->line_mat_host.data(), L * sizeof(int32_t), hipMemcpyHostToDevice, s)) !=
+          hipSuccess)
+    return done(hipfail(ctx, e, "hipMemcpy"));
+  if (facsz > 0)
+    hipLaunchKernelGGL(band_compact_kernel, dim3(ew_blocks(facsz)), dim3(EW_THREADS), 0, s, g, band, drep, b->nu,
+                       b->fac);
+  const int64_t ublocks = (b->nu + 63) / 64;
+  if (ublocks > 0) hipLaunchKernelGGL(band_lu_kernel, dim3(ublocks), dim3(64), 0, s, g, b->fac, b->nu);
+  if ((e = hipStreamSynchronize(s)) != hipSuccess) return done(hipfail(ctx, e, "banded factorisation"));
+  rc = done(ATM_OK);
+  if (rc) return rc;
   *out = b;
   return ATM_OK;
 }
 
 int atm_banded_destroy(atm_banded* b) {
   if (!b) return ATM_ERR_ARG;
-  if (b->band) hipFree(b->band);
+  if (b->fac) hipFree(b->fac);
+  if (b->line_mat) hipFree(b->line_mat);
   if (b->elem_col) hipFree(b->elem_col);
   if (b->elem_ez) hipFree(b->elem_ez);
   if (b->colelem) hipFree(b->colelem);
@@ -861,10 +1006,14 @@ int atm_banded_solve(atm_banded* b, double* x, const double* rhs) {
   atm_ctx* ctx = b->lin->ctx;
   const BandGeo g = band_geo(b);
   const int64_t blocks = (b->nlines + 63) / 64;
-  if (blocks > 0) hipLaunchKernelGGL(band_solve_kernel, dim3(blocks), dim3(64), 0, ctx->stream, g, b->band, x, rhs);
+  if (blocks > 0)
+    hipLaunchKernelGGL(band_solve_kernel, dim3(blocks), dim3(64), 0, ctx->stream, g, b->fac, b->line_mat, b->nu, x,
+                       rhs);
   HIPCHK(ctx, hipGetLastError());
   return ATM_OK;
 }
+
+int64_t atm_banded_nunique(const atm_banded* b) { return b ? b->nu : ATM_ERR_ARG; }
 
 int atm_banded_shape(const atm_banded* b, int64_t* nlines, int32_t* n, int32_t* bw) {
   if (!b) return ATM_ERR_ARG;
@@ -878,14 +1027,16 @@ int atm_banded_shape(const atm_banded* b, int64_t* nlines, int32_t* n, int32_t* 
 int atm_banded_download(atm_banded* b, double* host) {
   if (!b || !host) return ATM_ERR_ARG;
   atm_ctx* ctx = b->lin->ctx;
-  const size_t sz = (size_t)b->nlines * b->n * b->nb;
+  const size_t sz = (size_t)b->nu * b->n * b->nb;
   std::vector<double> tmp(sz);
   HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
-  HIPCHK(ctx, hipMemcpy(tmp.data(), b->band, sz * sizeof(double), hipMemcpyDeviceToHost));
-  for (int64_t line = 0; line < b->nlines; ++line)
+  HIPCHK(ctx, hipMemcpy(tmp.data(), b->fac, sz * sizeof(double), hipMemcpyDeviceToHost));
+  for (int64_t line = 0; line < b->nlines; ++line) {
+    const int64_t u = b->line_mat_host[line];
     for (int r = 0; r < b->n; ++r)
       for (int k = 0; k < b->nb; ++k)
-        host[((size_t)line * b->n + r) * b->nb + k] = tmp[((size_t)r * b->nb + k) * b->nlines + line];
+        host[((size_t)line * b->n + r) * b->nb + k] = tmp[((size_t)r * b->nb + k) * b->nu + u];
+  }
   return ATM_OK;
 }
 

@@ -56,6 +56,16 @@ __device__ __forceinline__ double lm_series(double u) {
 
 constexpr double LM_U0 = 0.05;
 
+// doubles per element of the LDS exchange buffer: pair shares (M*Ns per node)
+// and, afterwards, the face-point surface results (Ns per face point)
+template <class L, int NQ>
+constexpr int fast_xbuf() {
+  constexpr int D = L::D, Ns = L::Ns;
+  constexpr int Np = D == 1 ? NQ : (D == 2 ? NQ * NQ : NQ * NQ * NQ);
+  constexpr int a = (NQ / 2) * Ns * Np, b = 2 * D * (Np / NQ) * Ns;
+  return a > b ? a : b;
+}
+
 // Per-node primitives: [rho, u (D), b, u.u, phi, 1/b]
 template <int D, bool GRAV>
 struct FastEC {
@@ -239,6 +249,10 @@ ATM_DEV void fd_fast_body(const DevDG& p, double* lds, int64_t blk, const double
       }
     }
   }
+  // surface term: face points dealt out over the element's threads, results in LDS
+  __syncthreads();  // every thread has consumed its exchange slots
+  if (active) surface_faces_lds<L, SF>(p, Nq, Np, e, ijk, q, sx);
+  __syncthreads();
   if (!active) return;
   const double MJIn = p.MJI[node];
   double acc[Ns];
@@ -247,7 +261,7 @@ ATM_DEV void fd_fast_body(const DevDG& p, double* lds, int64_t blk, const double
     const double v = src[s] + MJIn * vol[s];
     acc[s] = assign ? v : alpha * dq[s * p.Ntot + node] + v;
   }
-  surface_node<L, SF>(p, Nq, e, idx, q, q1, a1, MJIn, acc);
+  surface_gather<L>(p, Nq, idx, sx, acc);
   epilogue<L>(p, node, q1, acc, dq, qout, bdt);
 }
 
@@ -261,9 +275,9 @@ constexpr int fast_epw() {
 
 template <class L, int NQ>
 size_t fast_lds_bytes() {
-  constexpr int D = L::D, Ns = L::Ns, NP = FastEC<L::D, true>::NP;
+  constexpr int D = L::D, NP = FastEC<L::D, true>::NP;
   constexpr int Np = D == 1 ? NQ : (D == 2 ? NQ * NQ : NQ * NQ * NQ);
-  return sizeof(double) * ((size_t)NQ * NQ + (size_t)fast_epw<L, NQ>() * (NP + D + (NQ / 2) * Ns) * Np);
+  return sizeof(double) * ((size_t)NQ * NQ + (size_t)fast_epw<L, NQ>() * ((NP + D) * Np + fast_xbuf<L, NQ>()));
 }
 
 template <class L, int NQ, int SF>

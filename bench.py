@@ -232,6 +232,29 @@ def vortex_workload(args):
                        "dof": dof, "us_per_stage": ms / (5 * args.steps) * 1e3, "precision": args.precision}}
 
 
+def serve_workload(args):
+    """Served stream: Poisson arrivals (rate 50/s, PCG64 seed 20260103) for --serve-seconds,
+    50/30/20 vortex (8..128^2, N 3..7) / shallow water (8..64^2, N 3..7) / IMEX Euler-gravity
+    (8..32^2 x 10, N 3) requests with U{50..500} steps, continuously batched."""
+    import atum_hip as A
+    from serve_requests import make_stream
+
+    reqs = make_stream(duration=args.serve_seconds)
+    server = A.Server(chunk=args.chunk, keep_results=False)
+    stats = server.run(reqs)
+    s = stats.summary()
+    kinds = {k: sum(r.kind == k for r in reqs) for k in ("vortex", "sw", "imex")}
+    return {"metric": "served stream: aggregate FP64 DOF-updates/s per RK stage, p50/p99 time-to-completion",
+            "value": s["dof_updates_per_s"], "unit": "DOF-updates/s", "n_gpus": 1, "steps": None,
+            "warmup": 0, "ms_per_step": None, "higher_is_better": True, "scaling": "weak",
+            "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+            "config": {"workload": f"Poisson 50/s for {args.serve_seconds} s, mix 50/30/20 "
+                                   "vortex/shallow-water/IMEX, steps U{50..500}", "requests": len(reqs),
+                       "kinds": kinds, "chunk_steps": args.chunk},
+            "p50_s": s["p50_s"], "p99_s": s["p99_s"], "mean_s": s["mean_s"], "completed": s["completed"],
+            "wall_s": s["wall_s"], "ticks": s["ticks"], "max_active": s["max_active"]}
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -247,9 +270,11 @@ def main():
                          "reference: the reference's summation order, no FMA")
     ap.add_argument("--no-reference-order", action="store_true",
                     help="skip the extra timing of the reference-order kernel")
-    ap.add_argument("--workload", choices=["bubble", "ensemble", "imex", "vortex"], default="bubble",
+    ap.add_argument("--workload", choices=["bubble", "ensemble", "imex", "vortex", "serve"], default="bubble",
                     help="bubble (default, the metric's configuration); the others are extra lines")
     ap.add_argument("--instances", type=int, default=256)
+    ap.add_argument("--serve-seconds", type=float, default=10.0)
+    ap.add_argument("--chunk", type=int, default=10, help="steps per serving tick")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build measures one GPU; the path shards by elements with one face "
@@ -262,7 +287,8 @@ def main():
     import atum_hip as A
 
     if args.workload != "bubble":
-        fn = {"ensemble": ensemble_workload, "imex": imex_workload, "vortex": vortex_workload}[args.workload]
+        fn = {"ensemble": ensemble_workload, "imex": imex_workload, "vortex": vortex_workload,
+              "serve": serve_workload}[args.workload]
         print(json.dumps(fn(args)))
         return
 

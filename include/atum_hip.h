@@ -184,6 +184,9 @@ int atm_banded_solve(atm_banded* b, double* x, const double* rhs);
 /* Copy the factored band to the host: lines x n x (2*bw+1) in [line][row][bw+col-row]. */
 int atm_banded_shape(const atm_banded* b, int64_t* nlines, int32_t* n, int32_t* bw);
 int atm_banded_download(atm_banded* b, double* host);
+/* Number of distinct line matrices kept (lines with bit-identical bands share one
+ * factor, e.g. every column of a horizontally homogeneous mesh). */
+int64_t atm_banded_nunique(const atm_banded* b);
 
 /* ---- ARK23 (src/odesolvers.jl:231-388) -------------------------------------
  * Replaces ARK23(rhs!, linrhs!, q, dt; split_rhs, paperversion) + solve!.

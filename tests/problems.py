@@ -205,3 +205,20 @@ def sw_bump(x, A, w, c):
 def sw_ensemble_dt(K, N, A, grav=9.81):
     """dt = (1/8) dx / N / sqrt(g h_max) (experiment/shallow_water/bickleyjet.jl:53-54 rule)."""
     return 0.125 * (1.0 / K) / N / np.sqrt(grav * (1 + A))
+
+
+def theta_noise_state(grid, aux, seed, amplitude=0.1, gamma=1.4, p_s=1e5):
+    """Isothermal background (aux rows dim+1, dim+2 = rho_ref, p_ref) at rest plus a
+    seeded N(0, amplitude^2) potential-temperature noise at fixed pressure."""
+    dim = grid.dim
+    rho0, p0 = aux[dim + 1], aux[dim + 2]
+    rng = np.random.Generator(np.random.PCG64(seed))
+    cp, cv = gamma * CV_D, CV_D
+    R = cp - cv
+    T0 = p0 / (rho0 * R)
+    exner = (p0 / p_s) ** (R / cp)
+    theta = T0 / exner + amplitude * rng.standard_normal(grid.Ntot)
+    T = theta * exner
+    rho = p0 / (R * T)
+    z = np.zeros_like(rho)
+    return np.stack([rho] + [z] * dim + [rho * (cv * T + aux[0])])

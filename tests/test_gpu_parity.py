@@ -334,3 +334,47 @@ def test_linear_euler_gravity_golden_on_gpu():
     err = A.weightednorm(dg, q.numpy() - q0)
     g = GOLD["linear_euler_gravity_2d"]
     assert err == pytest.approx(g["value"], rel=g["rtol"])
+
+
+@pytest.mark.parametrize("varying", [False, True])
+def test_banded_dedupe_and_solve(varying):
+    """Lines with bit-identical bands share one factor; with a horizontally varying
+    reference state every line keeps its own.  Solves match the oracle either way."""
+    import atum_hip as A
+    from oracle import oracle as O
+    dim, N = 3, 2
+    vx, vz = P.julia_range(0, 2000, 3), P.julia_range(0, 1000, 4)
+    grid = A.brickgrid(A.LobattoCell(N, dim), (vx, vx, vz), (True, True, False), "stacked")
+
+    def ref(x):
+        rho, p = P.isothermal_reference(x)
+        if varying:
+            rho = rho * (1 + 0.01 * x[0] / 2000 + 0.0037 * x[1] / 2000)
+        return rho, p
+
+    prob = A.Problem(boundary="reflect", referencestate=ref)
+    law = A.LinearEulerGravityLaw(A.EulerGravityLaw(dim, grav=9.81, problem=prob))
+    dg = A.DGSEM(law=law, grid=grid, surface_numericalflux=A.RusanovFlux(), directions=(dim,))
+    og = O.Grid(N, [vx, vx, vz], [True, True, False], stacked=True)
+    odg = O.DG("linear_euler_gravity", og, directions=(dim,), surface_flux="rusanov", bc="reflect",
+               aux=dg.auxstate, grav=9.81)
+    dt = 3.0
+    q0 = P.linear_test_initial(dg.auxstate)
+    qf = dg.fieldarray(q0)
+    ark = A.ARK23(None, dg, qf, dt)
+    nu = A._lib.lib.atm_banded_nunique(ark.band)
+    # a line's band is fixed by its horizontal node position (x, y): one factor per
+    # distinct position when the reference state varies horizontally, else one in all
+    colelem, KV, ncols = grid.columns()
+    Nh = grid.Np // grid.Nq
+    pos = {(grid.points[0, grid.Np * colelem[c * KV] + h], grid.points[1, grid.Np * colelem[c * KV] + h])
+           for c in range(ncols) for h in range(Nh)}
+    assert nu == (len(pos) if varying else 1)
+    assert len(pos) > 1
+    band_orc = odg.banded_factor(odg.banded_assemble(), -dt * (1 - 1 / math.sqrt(2)))
+    assert relmax(ark.band_host(), band_orc) <= 1e-12
+    b = np.random.default_rng(3).standard_normal(q0.shape)
+    bf = dg.fieldarray(b)
+    dg.ctx.check(A._lib.lib.atm_banded_solve(ark.band, bf.ptr, bf.ptr), "solve")
+    dg.ctx.sync()
+    assert relmax(bf.numpy(), odg.banded_solve(band_orc, b)) <= 1e-11
