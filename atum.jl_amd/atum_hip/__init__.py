@@ -16,8 +16,8 @@ from .grid import Grid, LobattoCell, brickgrid, lgl  # noqa: F401
 from .laws import (AdvectionLaw, CentralFlux, DummyProblem, EntropyConservativeFlux,  # noqa: F401
                    EulerGravityLaw, EulerLaw, KennedyGruberFlux, LinearEulerGravityLaw,
                    MatrixFlux, Problem, RoeFlux, RusanovFlux, ShallowWaterLaw)
-from .measures import weightednorm  # noqa: F401
-from .odesolvers import ARK23, LSRK54, LSRK144, solve_  # noqa: F401
+from .measures import entropyintegral, entropyproduct, weightednorm  # noqa: F401
+from .odesolvers import ARK23, LSRK54, LSRK144, RLSRK54, RLSRK144, solve_  # noqa: F401
 from .batch import Batch, LSRKEnsemble  # noqa: F401
 from .serve import Request, Server, run_alone  # noqa: F401
 

@@ -83,6 +83,10 @@ _sig("atm_ark23_run", c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, 
 _sig("atm_event_record", c_int, [c_void_p, c_int])
 _sig("atm_event_elapsed_ms", c_int, [c_void_p, c_int, c_int, ctypes.POINTER(ctypes.c_float)])
 _sig("atm_axpby", c_int, [c_void_p, c_void_p, c_void_p, c_double, c_double, c_int64])
+_sig("atm_entropy_integral", c_int, [c_void_p, c_void_p, _dp])
+_sig("atm_entropy_product", c_int, [c_void_p, c_void_p, c_void_p, _dp])
+_sig("atm_weightednorm", c_int, [c_void_p, c_void_p, c_double, c_int, _dp])
+_sig("atm_rlsrk_run", c_int, [c_void_p, c_int, c_void_p, c_void_p, c_double, c_double, c_int64, _dp])
 
 EXPORTED = [n for n in dir(lib) if n.startswith("atm_")]
 

@@ -96,6 +96,52 @@ class LSRK144(LSRK):
         return a, b, c
 
 
+class RLSRK:
+    """Relaxation LSRK (src/odesolvers.jl:69-136): after the LSRK stages the step
+    is scaled, q = q0 + gamma*(q - q0), with gamma the root near 1 of
+    eta(q0 + gamma*k) - eta0 - gamma*deta (deta the entropy production of the
+    stages), so that the entropy evolves as the semi-discretisation dictates.
+    Stages, entropy reductions and the Newton iteration all run on the device
+    (`atm_rlsrk_run`)."""
+
+    scheme = 0
+    nstages = 5
+
+    def __init__(self, rhs, q: Field, dt: float, t0: float = 0.0):
+        self.rhs = rhs
+        self.dt = float(dt)
+        self.time = float(t0)
+        self.gamma = 1.0
+        self.work = Field(3 * q.shape[0], q.shape[1], rhs.ctx).fill_(0.0)  # dq, q0, k
+
+    def run(self, q: Field, nsteps: int, dt: float | None = None):
+        dt = self.dt if dt is None else float(dt)
+        if nsteps <= 0:
+            return
+        g = ctypes.c_double(self.gamma)
+        self.rhs.ctx.check(_lib.lib.atm_rlsrk_run(self.rhs.h, self.scheme, q.ptr, self.work.ptr, dt,
+                                                  self.time, int(nsteps), ctypes.byref(g)),
+                           "atm_rlsrk_run")
+        self.gamma = g.value
+        for _ in range(nsteps):
+            self.time += dt
+
+    def dostep(self, q: Field, after_stage=None):
+        if after_stage is not None:
+            raise NotImplementedError("the device RLSRK step has no per-stage hook; use after_step")
+        self.run(q, 1)
+
+
+class RLSRK54(RLSRK):
+    scheme = 0
+    nstages = 5
+
+
+class RLSRK144(RLSRK):
+    scheme = 1
+    nstages = 14
+
+
 class ARK23:
     """3-stage IMEX ARK of Giraldo, Kelly and Constantinescu (src/odesolvers.jl:231-388).
 

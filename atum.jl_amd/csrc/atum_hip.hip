@@ -8,6 +8,7 @@
 #include <string>
 #include <vector>
 
+#include "measures.hpp"
 #include "resolve.hpp"
 
 using namespace atm;
@@ -30,6 +31,8 @@ struct atm_dg {
   int epw = 1;
   int law = 0, dim = 0, Nq = 0, Np = 0, Ns = 0;
   int64_t Ntot = 0;
+  MeasureFns mfn;
+  double* red = nullptr;  // reduction partials, relaxation scalars, results (allocated on first use)
   std::vector<void*> allocs;
 };
 
@@ -395,6 +398,37 @@ int upload_copy(atm_dg* dg, const T* host, size_t count, const T** out) {
   return ATM_OK;
 }
 
+// reduction buffer of an operator: partials, then RED_STATE relaxation slots, then RED_OUT results
+constexpr int64_t RED_PART = (int64_t)RED_MAXBLK * RED_MAXQ * 2;
+constexpr int64_t RED_STATE = 16;
+constexpr int64_t RED_OUT = 16;
+
+int red_buffer(atm_dg* dg) {
+  if (dg->red) return ATM_OK;
+  void* p = nullptr;
+  HIPCHK(dg->ctx, hipMalloc(&p, (RED_PART + RED_STATE + RED_OUT) * sizeof(double)));
+  dg->allocs.push_back(p);
+  dg->red = static_cast<double*>(p);
+  return ATM_OK;
+}
+
+double* red_state(atm_dg* dg) { return dg->red + RED_PART; }
+double* red_out(atm_dg* dg) { return dg->red + RED_PART + RED_STATE; }
+
+// reduce (mode) then combine into out[0..nq) and copy to the host
+int reduce_to_host(atm_dg* dg, int mode, const double* q, const double* k, double* host) {
+  atm_ctx* ctx = dg->ctx;
+  if (!dg->mfn.reduce) return fail(ctx, ATM_ERR_UNSUPPORTED, "the balance law defines no entropy");
+  int rc = red_buffer(dg);
+  if (rc) return rc;
+  hipStream_t s = ctx->stream;
+  HIPCHK(ctx, dg->mfn.reduce(dg->dev, s, mode, q, k, nullptr, red_state(dg), dg->red));
+  HIPCHK(ctx, launch_finalize(s, dg->red, red_blocks(dg->Ntot), 1, FIN_OUT, 0.0, red_state(dg), red_out(dg)));
+  HIPCHK(ctx, hipMemcpyAsync(host, red_out(dg), sizeof(double), hipMemcpyDeviceToHost, s));
+  HIPCHK(ctx, hipStreamSynchronize(s));
+  return ATM_OK;
+}
+
 int launch(atm_dg* dg, const double* q, double* dq, double alpha, int assign, double* qout, double bdt) {
   hipError_t e = dg->fn(dg->dev, dg->ctx->stream, q, dq, alpha, assign, qout, bdt);
   if (e != hipSuccess) return hipfail(dg->ctx, e, "element kernel launch");
@@ -559,6 +593,7 @@ int atm_dg_create(atm_ctx* ctx, const atm_dg_desc* d, atm_dg** out) {
   dg->bfn = r.bfn;
   dg->lds = r.lds;
   dg->epw = r.epw;
+  dg->mfn = resolve_measures(d->law, d->dim);
   DevDG& p = dg->dev;
   p.c.gamma = d->gamma;
   p.c.grav = d->grav;
@@ -706,6 +741,113 @@ int atm_axpby(atm_ctx* ctx, double* y, const double* x, double a, double b, int6
   if (!ctx || !y || !x) return ATM_ERR_ARG;
   hipLaunchKernelGGL(axpby_kernel, dim3(ew_blocks(n)), dim3(EW_THREADS), 0, ctx->stream, y, x, a, b, n);
   HIPCHK(ctx, hipGetLastError());
+  return ATM_OK;
+}
+
+// ---- measures (src/measures.jl) and relaxation LSRK ----------------------------
+
+int atm_entropy_integral(atm_dg* dg, const double* q, double* out) {
+  if (!dg || !q || !out) return fail(dg ? dg->ctx : nullptr, ATM_ERR_ARG, "atm_entropy_integral: null argument");
+  hipSetDevice(dg->ctx->device);
+  return reduce_to_host(dg, RED_ENTROPY, q, nullptr, out);
+}
+
+int atm_entropy_product(atm_dg* dg, const double* p, const double* q, double* out) {
+  if (!dg || !p || !q || !out) return fail(dg ? dg->ctx : nullptr, ATM_ERR_ARG, "atm_entropy_product: null argument");
+  hipSetDevice(dg->ctx->device);
+  return reduce_to_host(dg, RED_PRODUCT, p, q, out);
+}
+
+int atm_weightednorm(atm_dg* dg, const double* q, double p, int componentwise, double* out) {
+  if (!dg || !q || !out) return fail(dg ? dg->ctx : nullptr, ATM_ERR_ARG, "atm_weightednorm: null argument");
+  if (!(p > 0)) return fail(dg->ctx, ATM_ERR_ARG, "atm_weightednorm: p must be positive (or infinity)");
+  atm_ctx* ctx = dg->ctx;
+  hipSetDevice(ctx->device);
+  int rc = red_buffer(dg);
+  if (rc) return rc;
+  const int Ns = dg->Ns;
+  const bool inf = std::isinf(p);
+  hipStream_t s = ctx->stream;
+  HIPCHK(ctx, launch_wnorm(s, q, dg->dev.MJ, Ns, dg->Ntot, p, dg->red));
+  HIPCHK(ctx, launch_finalize(s, dg->red, red_blocks(dg->Ntot), Ns, inf ? FIN_MAX : FIN_OUT, 0.0, red_state(dg),
+                              red_out(dg)));
+  double v[RED_MAXQ];
+  HIPCHK(ctx, hipMemcpyAsync(v, red_out(dg), Ns * sizeof(double), hipMemcpyDeviceToHost, s));
+  HIPCHK(ctx, hipStreamSynchronize(s));
+  // per component (sum MJ |f|^p)^(1/p), then the p-norm of the component norms
+  for (int c = 0; c < Ns; ++c)
+    if (!inf) v[c] = (p == 2.0) ? std::sqrt(v[c]) : ((p == 1.0) ? v[c] : std::pow(v[c], 1.0 / p));
+  if (componentwise) {
+    for (int c = 0; c < Ns; ++c) out[c] = v[c];
+    return ATM_OK;
+  }
+  double tot = 0.0;
+  if (inf) {
+    for (int c = 0; c < Ns; ++c) tot = (std::fabs(v[c]) > tot || v[c] != v[c]) ? std::fabs(v[c]) : tot;
+  } else if (p == 2.0) {
+    for (int c = 0; c < Ns; ++c) tot += v[c] * v[c];
+    tot = std::sqrt(tot);
+  } else if (p == 1.0) {
+    for (int c = 0; c < Ns; ++c) tot += std::fabs(v[c]);
+  } else {
+    for (int c = 0; c < Ns; ++c) tot += std::pow(std::fabs(v[c]), p);
+    tot = std::pow(tot, 1.0 / p);
+  }
+  out[0] = tot;
+  return ATM_OK;
+}
+
+int atm_rlsrk_run(atm_dg* dg, int scheme, double* q, double* work, double dt, double t0, int64_t nsteps,
+                  double* gamma) {
+  (void)t0;
+  if (!dg || !q || !work || !gamma) return fail(dg ? dg->ctx : nullptr, ATM_ERR_ARG, "atm_rlsrk_run: null argument");
+  if (scheme != ATM_LSRK54 && scheme != ATM_LSRK144) return fail(dg->ctx, ATM_ERR_ARG, "unknown LSRK scheme");
+  if (!dg->mfn.reduce) return fail(dg->ctx, ATM_ERR_UNSUPPORTED, "the balance law defines no entropy");
+  atm_ctx* ctx = dg->ctx;
+  hipSetDevice(ctx->device);
+  int rc = red_buffer(dg);
+  if (rc) return rc;
+  const int64_t n = (int64_t)dg->Ns * dg->Ntot;
+  double* dq = work;
+  double* q0 = work + n;
+  double* k = work + 2 * n;
+  if (q >= work && q < work + 3 * n) return fail(ctx, ATM_ERR_ARG, "atm_rlsrk_run: q aliases the work buffer");
+  const LSRKTable t = lsrk_table(scheme);
+  // b coefficients of the equivalent standard RK method (src/odesolvers.jl:91-95)
+  double bfull[14];
+  bfull[t.ns - 1] = t.b[t.ns - 1];
+  for (int i = t.ns - 2; i >= 0; --i) bfull[i] = t.a[i + 1] * bfull[i + 1] + t.b[i];
+  hipStream_t s = ctx->stream;
+  double* st = red_state(dg);
+  const int nblk = red_blocks(dg->Ntot);
+  HIPCHK(ctx, launch_set_slot(s, st, RX_GAMMA, *gamma));
+  for (int64_t step = 0; step < nsteps; ++step) {
+    // q0 = q; eta0 = entropyintegral(q0); deta = -0.0
+    hipLaunchKernelGGL(copy_kernel, dim3(ew_blocks(n)), dim3(EW_THREADS), 0, s, q0, q, n);
+    HIPCHK(ctx, hipGetLastError());
+    HIPCHK(ctx, dg->mfn.reduce(dg->dev, s, RED_ENTROPY, q0, nullptr, nullptr, st, dg->red));
+    HIPCHK(ctx, launch_finalize(s, dg->red, nblk, 1, FIN_ETA0, 0.0, st, nullptr));
+    for (int i = 0; i < t.ns; ++i) {
+      if ((rc = launch(dg, q, k, 1.0, 1, nullptr, 0.0))) return rc;  // k = L(q), increment = false
+      HIPCHK(ctx, dg->mfn.stage(dg->dev, s, q, dq, k, t.a[i], t.b[i] * dt, dg->red));
+      HIPCHK(ctx, launch_finalize(s, dg->red, nblk, 1, FIN_DETA, bfull[i] * dt, st, nullptr));
+    }
+    HIPCHK(ctx, launch_diff(s, k, q, q0, n));  // k = q - q0
+    // Newton iterations on the device, four per check of the done flag
+    for (int chunk = 0; chunk < 13; ++chunk) {
+      for (int it = 0; it < 4; ++it) {
+        HIPCHK(ctx, dg->mfn.reduce(dg->dev, s, RED_RELAX, nullptr, k, q0, st, dg->red));
+        HIPCHK(ctx, launch_finalize(s, dg->red, nblk, 2, FIN_NEWTON, 0.0, st, nullptr));
+      }
+      double done = 0.0;
+      HIPCHK(ctx, hipMemcpyAsync(&done, st + RX_DONE, sizeof(double), hipMemcpyDeviceToHost, s));
+      HIPCHK(ctx, hipStreamSynchronize(s));
+      if (done != 0.0) break;
+    }
+    HIPCHK(ctx, launch_relax_apply(s, q, q0, k, st, n));
+  }
+  HIPCHK(ctx, hipMemcpyAsync(gamma, st + RX_GAMMA, sizeof(double), hipMemcpyDeviceToHost, s));
+  HIPCHK(ctx, hipStreamSynchronize(s));
   return ATM_OK;
 }
 

@@ -13,6 +13,7 @@
 // Epilogue: dq = (assign ? 0 : alpha*dq) + volume, then the surface terms are
 // subtracted, then optionally q_out = q + bdt*dq (src/odesolvers.jl:51-53).
 #pragma once
+#include "fast_flux.hpp"
 #include "pairflux.hpp"
 
 namespace atm {
@@ -122,7 +123,7 @@ ATM_DEV void surface_node(const DevDG& p, int Nq, int64_t e, const int idx[3], c
 // points (faces of the directions in the mask), each writes faceMJ*nf*MJI of
 // its face point to buf[s * nfp + f], f = face*Nfpf + fp; surface_gather then
 // subtracts a node's slots in direction order, as the node-centric form does.
-template <class L, int SF>
+template <class L, int SF, bool FAST>
 ATM_DEV void surface_faces_lds(const DevDG& p, int Nq, int Np, int64_t e, int t, const double* q, double* buf) {
   constexpr int D = L::D, Ns = L::Ns, NA = L::NA;
   const int Nfpf = p.Nfpf, nfp = 2 * D * Nfpf;
@@ -158,7 +159,8 @@ ATM_DEV void surface_faces_lds(const DevDG& p, int Nq, int Np, int64_t e, int t,
 #pragma unroll
       for (int s = 0; s < NA; ++s) ap[s] = am[s];
     }
-    dispatch_sflux<L, SF>(p, n, qm, am, qp, ap, nf);
+    if constexpr (FAST && has_fast_sflux<L, SF>()) fast_sflux<L, SF>(p.c, n, qm, am, qp, ap, nf);
+    else dispatch_sflux<L, SF>(p, n, qm, am, qp, ap, nf);
     const double MJI = p.MJI[node];
 #pragma unroll
     for (int s = 0; s < Ns; ++s) buf[s * nfp + f] = fMJ * nf[s] * MJI;

@@ -251,7 +251,7 @@ ATM_DEV void fd_fast_body(const DevDG& p, double* lds, int64_t blk, const double
   }
   // surface term: face points dealt out over the element's threads, results in LDS
   __syncthreads();  // every thread has consumed its exchange slots
-  if (active) surface_faces_lds<L, SF>(p, Nq, Np, e, ijk, q, sx);
+  if (active) surface_faces_lds<L, SF, true>(p, Nq, Np, e, ijk, q, sx);
   __syncthreads();
   if (!active) return;
   const double MJIn = p.MJI[node];

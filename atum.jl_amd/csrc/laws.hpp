@@ -80,6 +80,10 @@ struct Advection {
   ATM_DEV static double wavespeed(const LawConst& c, const double* n, const double*, const double*) {
     return fabs(dotv<D>(n, c.u));
   }
+  // advection.jl:18-19
+  static constexpr bool HAS_ENTROPY = true;
+  ATM_DEV static double entropy(const LawConst&, const double* q, const double*) { return q[0] * q[0] / 2; }
+  ATM_DEV static void entropyvariables(const LawConst&, const double* q, const double*, double* v) { v[0] = q[0]; }
   template <int VF>
   ATM_DEV static void twopoint(const LawConst& c, const double* q1, const double* a1,
                                const double* q2, const double* a2, double F[D][Ns]);
@@ -120,6 +124,14 @@ struct Euler {
     for (int d = 0; d < D; ++d) u[d] = q[1 + d] / rho;
     const double p = pressure(c, rho, q + 1, q[D + 1]);
     return fabs(dotv<D>(n, u)) + sqrt(c.gamma * p / rho);
+  }
+  // euler.jl:68-75
+  static constexpr bool HAS_ENTROPY = true;
+  ATM_DEV static double entropy(const LawConst& c, const double* q, const double*) {
+    const double rho = q[0];
+    const double p = pressure(c, rho, q + 1, q[D + 1]);
+    const double s = log(p / pow(rho, c.gamma));
+    return -rho * s / (c.gamma - 1);
   }
   ATM_DEV static void entropyvariables(const LawConst& c, const double* q, const double*, double* v) {
     const double rho = q[0], g = c.gamma;
@@ -183,6 +195,14 @@ struct EulerGravity {
     for (int d = 0; d < D; ++d) u[d] = q[1 + d] / rho;
     const double p = pressure(c, rho, q + 1, q[D + 1], aux[0]);
     return fabs(dotv<D>(n, u)) + sqrt(c.gamma * p / rho);
+  }
+  // euler_gravity.jl:124-132
+  static constexpr bool HAS_ENTROPY = true;
+  ATM_DEV static double entropy(const LawConst& c, const double* q, const double* aux) {
+    const double rho = q[0];
+    const double p = pressure(c, rho, q + 1, q[D + 1], aux[0]);
+    const double s = log(p / pow(rho, c.gamma));
+    return -rho * s / (c.gamma - 1);
   }
   ATM_DEV static void entropyvariables(const LawConst& c, const double* q, const double* aux, double* v) {
     const double rho = q[0], g = c.gamma, phi = aux[0];
@@ -255,6 +275,13 @@ struct LinearEulerGravity {
   ATM_DEV static double wavespeed(const LawConst& c, const double*, const double*, const double* aux) {
     return sqrt(c.gamma * aux[D + 2] / aux[D + 1]);
   }
+  // the linearized law defines no entropy (linear_euler_gravity.jl)
+  static constexpr bool HAS_ENTROPY = false;
+  ATM_DEV static double entropy(const LawConst&, const double*, const double*) { return 0.0; }
+  ATM_DEV static void entropyvariables(const LawConst&, const double*, const double*, double* v) {
+#pragma unroll
+    for (int s = 0; s < Ns; ++s) v[s] = 0.0;
+  }
   template <int VF>
   ATM_DEV static void twopoint(const LawConst& c, const double* q1, const double* a1,
                                const double* q2, const double* a2, double F[D][Ns]);
@@ -301,6 +328,22 @@ struct ShallowWater {
 #pragma unroll
     for (int d = 0; d < D; ++d) u[d] = q[1 + d] / rho;
     return fabs(dotv<D>(n, u)) + sqrt(c.grav * rho);
+  }
+  // shallow_water.jl:52-66
+  static constexpr bool HAS_ENTROPY = true;
+  ATM_DEV static double entropy(const LawConst& c, const double* q, const double*) {
+    const double rho = q[0];
+    return dotv<D>(q + 1, q + 1) / (2 * rho) + c.grav * (rho * rho) / 2;
+  }
+  ATM_DEV static void entropyvariables(const LawConst& c, const double* q, const double*, double* v) {
+    const double rho = q[0];
+    double mr[D];
+#pragma unroll
+    for (int d = 0; d < D; ++d) mr[d] = -q[1 + d];
+    v[0] = dotv<D>(mr, q + 1) / (2 * (rho * rho)) + c.grav * rho;
+#pragma unroll
+    for (int d = 0; d < D; ++d) v[1 + d] = q[1 + d] / rho;
+    v[D + 1] = 0.0;
   }
   template <int VF>
   ATM_DEV static void twopoint(const LawConst& c, const double* q1, const double* a1,

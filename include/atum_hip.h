@@ -222,6 +222,31 @@ int atm_event_elapsed_ms(atm_ctx* ctx, int slot_start, int slot_end, float* ms);
 /* y = a*y + b*x over n doubles */
 int atm_axpby(atm_ctx* ctx, double* y, const double* x, double a, double b, int64_t n);
 
+/* ---- measures (src/measures.jl:6-33) ---------------------------------------
+ * Reductions over all nodes on the device, in double-double arithmetic with a
+ * fixed order (reproducible bit for bit); the result is copied to the host and
+ * the call returns after the stream has reached it.
+ * atm_entropy_integral: sum MJ*entropy(law, q, aux)   (entropyintegral(dg, q))
+ * atm_entropy_product:  sum MJ*entropyvariables(law, p, aux).q
+ *                                                     (entropyproduct(dg, p, q))
+ * ATM_ERR_UNSUPPORTED for a law without entropy (LinearEulerGravity). */
+int atm_entropy_integral(atm_dg* dg, const double* q, double* out);
+int atm_entropy_product(atm_dg* dg, const double* p, const double* q, double* out);
+/* weightednorm(dg, q, p; componentwise) (src/measures.jl:6-21): p > 0 or
+ * INFINITY; out holds Ns values if componentwise, else one. */
+int atm_weightednorm(atm_dg* dg, const double* q, double p, int componentwise, double* out);
+
+/* ---- relaxation LSRK (src/odesolvers.jl:69-146) ----------------------------
+ * Replaces RLSRK54/RLSRK144(rhs!, q, dt) + nsteps calls of dostep!.  Each step
+ * runs the LSRK stages with the stage entropy production accumulated on the
+ * device, then solves eta(q0 + gamma*k) - eta0 - gamma*deta = 0 for gamma by
+ * Newton's method on the device, starting from *gamma (the previous step's;
+ * 1 for a new solver), and sets q = q0 + gamma*k.  work is a device buffer of
+ * 3*Ns*Np*K doubles (dq, q0, k); dq must be finite (zero for a new solver).
+ * *gamma returns the last step's relaxation factor. */
+int atm_rlsrk_run(atm_dg* dg, int scheme, double* q, double* work, double dt, double t0, int64_t nsteps,
+                  double* gamma);
+
 #ifdef __cplusplus
 }
 #endif
