@@ -17,8 +17,9 @@ from .laws import (AdvectionLaw, CentralFlux, DummyProblem, EntropyConservativeF
                    EulerGravityLaw, EulerLaw, KennedyGruberFlux, LinearEulerGravityLaw,
                    MatrixFlux, Problem, RoeFlux, RusanovFlux, ShallowWaterLaw)
 from .measures import entropyintegral, entropyproduct, weightednorm  # noqa: F401
-from .odesolvers import ARK23, LSRK54, LSRK144, RLSRK54, RLSRK144, solve_  # noqa: F401
+from .odesolvers import ARK23, LSRK54, LSRK144, RLSRK54, RLSRK144, BandedOperator, solve_  # noqa: F401
 from .batch import Batch, LSRKEnsemble  # noqa: F401
 from .serve import Request, Server, run_alone  # noqa: F401
+from .snapshots import SnapshotWriter, read_snapshot, write_snapshot  # noqa: F401
 
 __all__ = [n for n in dir() if not n.startswith("_")]

@@ -8,6 +8,8 @@ from __future__ import annotations
 import ctypes
 import os
 
+import numpy as np
+
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.environ.get("ATUM_HIP_LIB",
                           os.path.join(os.path.dirname(_HERE), "build", "libatumhip.so"))
@@ -44,6 +46,18 @@ class DgDesc(ctypes.Structure):
                 ("aux", _dp), ("faceix_m", _i64p), ("faceix_p", _i64p), ("bctag", _i32p)]
 
 
+class BrickDesc(ctypes.Structure):
+    _fields_ = [("Kd", c_int32 * 3), ("periodic", c_int32 * 3), ("stacked", c_int32),
+                ("coords", _dp * 3), ("xi", _dp), ("w", _dp)]
+
+
+# atm_dg_geometry arrays: id, dtype, rows (1, dim, dim*dim), per (node, face, tag)
+GEOMETRY = {"points": (0, np.float64, "dim", "node"), "g": (1, np.float64, "dim2", "node"),
+            "MJ": (2, np.float64, "1", "node"), "MJI": (3, np.float64, "1", "node"),
+            "faceMJ": (4, np.float64, "1", "face"), "normal": (5, np.float64, "dim", "face"),
+            "faceix_p": (6, np.int64, "1", "face"), "bctag": (7, np.int32, "1", "tag")}
+
+
 def _sig(name, res, args):
     f = getattr(lib, name)
     f.restype = res
@@ -64,6 +78,9 @@ _sig("atm_upload", c_int, [c_void_p, c_void_p, c_void_p, ctypes.c_size_t])
 _sig("atm_download", c_int, [c_void_p, c_void_p, c_void_p, ctypes.c_size_t])
 _sig("atm_copy", c_int, [c_void_p, c_void_p, c_void_p, ctypes.c_size_t])
 _sig("atm_dg_create", c_int, [c_void_p, ctypes.POINTER(DgDesc), ctypes.POINTER(c_void_p)])
+_sig("atm_dg_create_brick", c_int, [c_void_p, ctypes.POINTER(DgDesc), ctypes.POINTER(BrickDesc),
+                                    ctypes.POINTER(c_void_p)])
+_sig("atm_dg_geometry", c_int, [c_void_p, c_int, c_void_p])
 _sig("atm_dg_destroy", c_int, [c_void_p])
 _sig("atm_dg_nstates", c_int, [c_void_p])
 _sig("atm_dg_nnodes", c_int64, [c_void_p])

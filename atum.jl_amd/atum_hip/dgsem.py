@@ -54,10 +54,13 @@ class FluxDifferencingForm:
 
 class DGSEM:
     def __init__(self, *, law, grid, surface_numericalflux, volume_form=None, directions=None,
-                 auxstate=None, precision="reference", ctx=None):
+                 auxstate=None, precision="reference", geometry="host", ctx=None):
         """precision: "reference" keeps the reference's summation order with every
         product and sum rounded; "fast" evaluates symmetric flux pairs once with FMA
-        (same operator, round-off-level differences)."""
+        (same operator, round-off-level differences).
+        geometry: "host" copies the grid's geometry arrays to the device; "device"
+        has the library build them from the brick's vertex coordinates
+        (atm_dg_create_brick), bit-identical, without forming them on the host."""
         self.ctx = ctx or _lib.context()
         self.law = law
         self.grid = grid
@@ -69,10 +72,10 @@ class DGSEM:
         self.directions = tuple(directions) if directions is not None else tuple(range(1, dim + 1))
         if not self.directions or any(d < 1 or d > dim for d in self.directions):
             raise ValueError("directions must be a non-empty subset of 1:dim")
-        # src/dgsem.jl:41-51
-        self.MJ = grid.MJ
-        self.MJI = grid.MJI
-        self.faceMJ = grid.faceMJ
+        if geometry not in ("host", "device"):
+            raise ValueError("geometry must be 'host' or 'device'")
+        self.geometry = geometry
+        self._geom = {}
         if auxstate is None:
             auxstate = law.auxiliary(grid.points)
         self.auxstate = None if auxstate is None else np.ascontiguousarray(auxstate, dtype=np.float64)
@@ -111,19 +114,56 @@ class DGSEM:
             st = np.ascontiguousarray(prob.bc_states, dtype=np.float64)
             d.n_bc_states = st.shape[0]
             d.bc_states = ptr(st)
-        d.D = ptr(grid.D.ravel(order="F"))
-        d.g, d.MJ, d.MJI = ptr(grid.g), ptr(grid.MJ), ptr(grid.MJI)
-        d.faceMJ, d.normal = ptr(grid.faceMJ), ptr(grid.normal)
+        d.D = ptr(np.ascontiguousarray(grid.cell.D).ravel(order="F"))
         if naux:
             d.aux = ptr(self.auxstate)
-        d.faceix_m = ptr(grid.faceix_m, ctypes.c_int64)
-        d.faceix_p = ptr(grid.faceix_p, ctypes.c_int64)
-        d.bctag = ptr(grid.bctag, ctypes.c_int32)
         h = ctypes.c_void_p()
-        self.ctx.check(_lib.lib.atm_dg_create(self.ctx.h, ctypes.byref(d), ctypes.byref(h)),
-                       "atm_dg_create")
+        if geometry == "device":
+            b = _lib.BrickDesc()
+            for i in range(dim):
+                b.Kd[i] = grid.Kd[i]
+                b.periodic[i] = int(grid.periodic[i])
+                b.coords[i] = ptr(np.asarray(grid.coords[i], dtype=np.float64))
+            b.stacked = int(grid.ordering == "stacked")
+            b.xi, b.w = ptr(grid.cell.points_1d), ptr(grid.cell.weights_1d)
+            self.ctx.check(_lib.lib.atm_dg_create_brick(self.ctx.h, ctypes.byref(d), ctypes.byref(b),
+                                                        ctypes.byref(h)), "atm_dg_create_brick")
+        else:
+            d.g, d.MJ, d.MJI = ptr(grid.g), ptr(grid.MJ), ptr(grid.MJI)
+            d.faceMJ, d.normal = ptr(grid.faceMJ), ptr(grid.normal)
+            d.faceix_m = ptr(grid.faceix_m, ctypes.c_int64)
+            d.faceix_p = ptr(grid.faceix_p, ctypes.c_int64)
+            d.bctag = ptr(grid.bctag, ctypes.c_int32)
+            self.ctx.check(_lib.lib.atm_dg_create(self.ctx.h, ctypes.byref(d), ctypes.byref(h)),
+                           "atm_dg_create")
         self.h = h
         self.Ns = _lib.lib.atm_dg_nstates(h)
+
+    # src/dgsem.jl:41-51: MJ, MJI, faceMJ (downloaded once when built on the device)
+    @property
+    def MJ(self):
+        return self.grid.MJ if self.geometry == "host" else self.geometry_array("MJ")
+
+    @property
+    def MJI(self):
+        return self.grid.MJI if self.geometry == "host" else self.geometry_array("MJI")
+
+    @property
+    def faceMJ(self):
+        return self.grid.faceMJ if self.geometry == "host" else self.geometry_array("faceMJ")
+
+    def geometry_array(self, name):
+        """A geometry array as held by the library (atm_dg_geometry): points, g, MJ,
+        MJI, faceMJ, normal, faceix_p or bctag."""
+        if name not in self._geom:
+            which, dtype, rows, per = _lib.GEOMETRY[name]
+            dim = self.grid.dim
+            n = (self.grid.Ntot if per == "node" else
+                 self.grid.K * self.grid.Nfp if per == "face" else 2 * dim * self.grid.K)
+            out = np.empty({"1": 1, "dim": dim, "dim2": dim * dim}[rows] * n, dtype=dtype)
+            self.ctx.check(_lib.lib.atm_dg_geometry(self.h, which, out.ctypes.data), "atm_dg_geometry")
+            self._geom[name] = out.reshape(-1, n) if rows != "1" else out
+        return self._geom[name]
 
     def __call__(self, dq: Field, q: Field, time: float = 0.0, increment: bool = True):
         self.ctx.check(_lib.lib.atm_dg_tendency(self.h, dq.ptr, q.ptr, float(time), int(increment)),

@@ -77,9 +77,16 @@ class LobattoCell:
 
 
 class Grid:
-    """A brick of affine hexahedral (quad, interval) elements with face tables."""
+    """A brick of affine hexahedral (quad, interval) elements with face tables.
 
-    def __init__(self, cell: LobattoCell, coordinates, periodic, ordering="cartesian"):
+    With lazy=True nothing is computed up front: the node coordinates are
+    formed on first use of `points`, the geometry and face tables on first use
+    of any of them (an operator built with geometry="device" never needs them
+    on the host)."""
+
+    _GEOMETRY = ("D", "elem_index", "MJ", "MJI", "g", "faceix_m", "faceix_p", "faceMJ", "normal", "bctag")
+
+    def __init__(self, cell: LobattoCell, coordinates, periodic, ordering="cartesian", lazy=False):
         dim = cell.dim
         if len(coordinates) != dim or len(periodic) != dim:
             raise ValueError("one coordinate array and periodic flag per dimension")
@@ -97,7 +104,18 @@ class Grid:
         self.Nfpf = self.Np // self.Nq
         self.Nfp = 2 * dim * self.Nfpf
         self.Ntot = self.Np * self.K
-        self._build()
+        if not lazy:
+            self._build()
+
+    def __getattr__(self, name):
+        # only called for attributes not yet set: build them on first use
+        if name == "points":
+            self._build_points()
+            return self.__dict__["points"]
+        if name in Grid._GEOMETRY:
+            self._build()
+            return self.__dict__[name]
+        raise AttributeError(name)
 
     # element id of element multi-indices (arrays)
     def elem_id(self, em):
@@ -116,28 +134,45 @@ class Grid:
                 stride *= K[d]
         return em[V] + K[V] * h
 
+    def _elements(self):
+        """Element multi-indices in element-id order, (dim, K)."""
+        if "_em" not in self.__dict__:
+            dim, K = self.dim, self.K
+            grids = np.meshgrid(*[np.arange(k) for k in self.Kd], indexing="ij")
+            em_all = [g.ravel() for g in grids]
+            eid = self.elem_id(em_all)
+            em = [np.empty(K, dtype=np.int64) for _ in range(dim)]
+            for d in range(dim):
+                em[d][eid] = em_all[d]
+            self._em = em
+        return self._em
+
+    def _build_points(self):
+        dim, Nq = self.dim, self.Nq
+        xi = self.cell.points_1d
+        em = self._elements()
+        nidx = np.indices((Nq,) * dim).reshape(dim, -1)[::-1]  # nidx[d][ijk], first dim fastest
+        points = np.empty((dim, self.Ntot))
+        for d in range(dim):
+            lo, hi = self.coords[d][em[d]], self.coords[d][em[d] + 1]
+            t = xi[nidx[d]]
+            x = (1 - t)[:, None] / 2 * lo[None, :] + (1 + t)[:, None] / 2 * hi[None, :]
+            points[d] = x.T.ravel()
+        self.points = points
+
     def _build(self):
         dim, Nq, Np, K = self.dim, self.Nq, self.Np, self.K
         xi, w, D = self.cell.points_1d, self.cell.weights_1d, self.cell.D
         self.D = np.ascontiguousarray(D)
-        # element multi-indices in element-id order
-        grids = np.meshgrid(*[np.arange(k) for k in self.Kd], indexing="ij")
-        em_all = [g.ravel() for g in grids]
-        eid = self.elem_id(em_all)
-        em = [np.empty(K, dtype=np.int64) for _ in range(dim)]
-        for d in range(dim):
-            em[d][eid] = em_all[d]
+        em = self._elements()
         self.elem_index = np.stack(em)  # (dim, K)
         # node multi-indices within an element
         nidx = np.indices((Nq,) * dim).reshape(dim, -1)[::-1]  # nidx[d][ijk], first dim fastest
         lo = np.stack([self.coords[d][em[d]] for d in range(dim)])       # (dim, K)
         hi = np.stack([self.coords[d][em[d] + 1] for d in range(dim)])
         h = hi - lo
-        self.points = np.empty((dim, self.Ntot))
-        for d in range(dim):
-            t = xi[nidx[d]]
-            x = (1 - t)[:, None] / 2 * lo[d][None, :] + (1 + t)[:, None] / 2 * hi[d][None, :]
-            self.points[d] = x.T.ravel()
+        if "points" not in self.__dict__:
+            self._build_points()
         J = np.prod(h / 2, axis=0)
         wp = np.ones(Np)
         for d in range(dim):
@@ -200,7 +235,7 @@ class Grid:
     def columns(self):
         """(colelem, KV, ncols): elements of each vertical column, bottom to top."""
         KV = self.Kd[-1]
-        em = self.elem_index
+        em = np.stack(self._elements())
         ncols = self.K // KV
         col = np.zeros(self.K, dtype=np.int64)
         stride = 1
@@ -217,7 +252,7 @@ class Grid:
         return min(np.min(np.diff(self.coords[d])) / 2 * gap for d in dims)
 
 
-def brickgrid(cell: LobattoCell, coordinates, periodic=None, ordering="cartesian") -> Grid:
+def brickgrid(cell: LobattoCell, coordinates, periodic=None, ordering="cartesian", lazy=False) -> Grid:
     """Bennu-style `brickgrid(cell, coordinates; periodic, ordering)`."""
     periodic = periodic if periodic is not None else (False,) * cell.dim
-    return Grid(cell, coordinates, periodic, ordering)
+    return Grid(cell, coordinates, periodic, ordering, lazy=lazy)

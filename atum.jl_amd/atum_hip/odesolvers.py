@@ -142,32 +142,81 @@ class RLSRK144(RLSRK):
     nstages = 14
 
 
+class BandedOperator:
+    """Per-line LU factors of A = I + scale*L for a vertical linear DGSEM operator L
+    (Bennu's batchedbandedmatrix + batchedbandedlu!, src/odesolvers.jl:264-268),
+    built on the device by `atm_banded_create`; lines with bit-identical bands share
+    one factor.  Several solvers with the same operator and scale may share it."""
+
+    def __init__(self, linrhs, scale: float):
+        self.linrhs = linrhs
+        self.scale = float(scale)
+        colelem, KV, ncols = linrhs.grid.columns()
+        self._colelem = np.ascontiguousarray(colelem, dtype=np.int32)
+        h = ctypes.c_void_p()
+        linrhs.ctx.check(_lib.lib.atm_banded_create(
+            linrhs.h, KV, self._colelem.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)), ncols,
+            self.scale, ctypes.byref(h)), "atm_banded_create")
+        self.h = h
+
+    @property
+    def nunique(self):
+        return int(_lib.lib.atm_banded_nunique(self.h))
+
+    def solve(self, x: Field, rhs: Field):
+        """x = A \\ rhs (x may be rhs)."""
+        self.linrhs.ctx.check(_lib.lib.atm_banded_solve(self.h, x.ptr, rhs.ptr), "atm_banded_solve")
+
+    def host(self):
+        """The factors of every line, (nlines, n, 2bw+1), band storage."""
+        n = ctypes.c_int32(0)
+        bw = ctypes.c_int32(0)
+        nl = ctypes.c_int64(0)
+        _lib.lib.atm_banded_shape(self.h, ctypes.byref(nl), ctypes.byref(n), ctypes.byref(bw))
+        out = np.empty((nl.value, n.value, 2 * bw.value + 1))
+        self.linrhs.ctx.check(_lib.lib.atm_banded_download(
+            self.h, out.ctypes.data_as(ctypes.POINTER(ctypes.c_double))), "atm_banded_download")
+        return out
+
+    def __del__(self):
+        try:
+            if getattr(self, "h", None):
+                _lib.lib.atm_banded_destroy(self.h)
+                self.h = None
+        except Exception:
+            pass
+
+
 class ARK23:
     """3-stage IMEX ARK of Giraldo, Kelly and Constantinescu (src/odesolvers.jl:231-388).
 
     `rhs` (explicit, may be None) and `linrhs` (the vertical linear operator)
     are DGSEM operators; the linear part is solved per vertical node line with
-    a banded LU of I - dt*(1 - 1/sqrt 2)*L built once at construction.
+    a banded LU of I - dt*(1 - 1/sqrt 2)*L built once at construction (or
+    `banded`, a BandedOperator of that operator and scale, shared with other
+    solvers).
     """
 
     def __init__(self, rhs, linrhs, q: Field, dt: float, t0: float = 0.0, paperversion=False,
-                 split_rhs=True):
+                 split_rhs=True, banded: BandedOperator | None = None):
         if linrhs is None:
             raise NotImplementedError("ARK23 without a linear operator is plain explicit RK")
         self.rhs, self.linrhs = rhs, linrhs
         self.dt = self.dt_fac = float(dt)
         self.time = float(t0)
         self.paperversion, self.split_rhs = bool(paperversion), bool(split_rhs)
-        ctx = linrhs.ctx
-        colelem, KV, ncols = linrhs.grid.columns()
-        self._colelem = np.ascontiguousarray(colelem, dtype=np.int32)
-        gamma = 1 - 1 / math.sqrt(2)
-        h = ctypes.c_void_p()
-        ctx.check(_lib.lib.atm_banded_create(
-            linrhs.h, KV, self._colelem.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)), ncols,
-            -self.dt_fac * gamma, ctypes.byref(h)), "atm_banded_create")
-        self.band = h
-        self.work = Field(9 * q.shape[0], q.shape[1], ctx)
+        scale = -self.dt_fac * (1 - 1 / math.sqrt(2))
+        if banded is None:
+            banded = BandedOperator(linrhs, scale)
+        elif banded.linrhs is not linrhs or banded.scale != scale:
+            raise ValueError("the banded operator was built for another operator or time step")
+        self.banded = banded
+        self.work = Field(9 * q.shape[0], q.shape[1], linrhs.ctx)
+
+    @property
+    def band(self):
+        """The library handle of the banded factors."""
+        return self.banded.h
 
     def run(self, q: Field, nsteps: int, dt: float | None = None):
         dt = self.dt if dt is None else float(dt)
@@ -183,22 +232,7 @@ class ARK23:
         self.run(q, 1)
 
     def band_host(self):
-        n = ctypes.c_int32(0)
-        bw = ctypes.c_int32(0)
-        nl = ctypes.c_int64(0)
-        _lib.lib.atm_banded_shape(self.band, ctypes.byref(nl), ctypes.byref(n), ctypes.byref(bw))
-        out = np.empty((nl.value, n.value, 2 * bw.value + 1))
-        self.linrhs.ctx.check(_lib.lib.atm_banded_download(
-            self.band, out.ctypes.data_as(ctypes.POINTER(ctypes.c_double))), "atm_banded_download")
-        return out
-
-    def __del__(self):
-        try:
-            if getattr(self, "band", None):
-                _lib.lib.atm_banded_destroy(self.band)
-                self.band = None
-        except Exception:
-            pass
+        return self.banded.host()
 
 
 def solve_(q: Field, timeend: float, solver, after_step=None, after_stage=None,

@@ -13,6 +13,7 @@ cost one upload.
 """
 from __future__ import annotations
 
+import math
 import time
 from dataclasses import dataclass, field
 from typing import Callable, Optional
@@ -22,7 +23,7 @@ import numpy as np
 from . import _lib
 from .batch import Batch
 from .fields import Field
-from .odesolvers import ARK23, LSRK54
+from .odesolvers import ARK23, LSRK54, BandedOperator
 
 
 @dataclass
@@ -83,7 +84,11 @@ class Server:
             ex, lin, q0, dt = built
             r._dg = lin
             r._q = lin.fieldarray(q0)
-            r._ark = ARK23(ex, lin, r._q, dt, split_rhs=False)
+            # requests on the same mesh with the same time step share one factorisation
+            bkey = ("banded", r.key, dt)
+            if bkey not in self.cache:
+                self.cache[bkey] = BandedOperator(lin, -dt * (1 - 1 / math.sqrt(2)))
+            r._ark = ARK23(ex, lin, r._q, dt, split_rhs=False, banded=self.cache[bkey])
         else:
             dg, q0, dt = built
             r._dg = dg

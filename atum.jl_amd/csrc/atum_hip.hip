@@ -8,33 +8,9 @@
 #include <string>
 #include <vector>
 
-#include "measures.hpp"
-#include "resolve.hpp"
+#include "handles.hpp"
 
 using namespace atm;
-
-struct atm_ctx {
-  int device = 0;
-  hipStream_t own = nullptr;
-  hipStream_t stream = nullptr;
-  size_t max_lds = 65536;
-  std::string err;
-  hipEvent_t events[16] = {};
-};
-
-struct atm_dg {
-  atm_ctx* ctx = nullptr;
-  DevDG dev{};
-  LaunchFn fn = nullptr;
-  BatchLaunchFn bfn = nullptr;
-  size_t lds = 0;
-  int epw = 1;
-  int law = 0, dim = 0, Nq = 0, Np = 0, Ns = 0;
-  int64_t Ntot = 0;
-  MeasureFns mfn;
-  double* red = nullptr;  // reduction partials, relaxation scalars, results (allocated on first use)
-  std::vector<void*> allocs;
-};
 
 struct atm_batch {
   atm_ctx* ctx = nullptr;
@@ -55,30 +31,18 @@ struct atm_banded {
   int n = 0, bw = 0, nb = 0, KV = 0, ncols = 0, Nh = 0;
   int64_t nlines = 0;
   int64_t nu = 0;               // distinct line matrices
-  double* fac = nullptr;        // LU factors of the distinct matrices: [(r*nb + bw + c - r) * nu + u]
+  double* fac = nullptr;        // LU factors of the distinct matrices: [u * n*nb + r*nb + bw + c - r]
   int32_t* line_mat = nullptr;  // matrix of each line
   std::vector<int32_t> line_mat_host;
+  int32_t* perm = nullptr;      // lines ordered by matrix, each matrix's run padded to 64 with -1
+  int32_t* wave_mat = nullptr;  // matrix of each run of 64
+  int64_t nwaves = 0;
   int32_t* elem_col = nullptr;  // column of each element (-1: none)
   int32_t* elem_ez = nullptr;   // position of each element in its column
   int32_t* colelem = nullptr;   // ncols*KV
 };
 
 namespace {
-
-int fail(atm_ctx* ctx, int code, const std::string& msg) {
-  if (ctx) ctx->err = msg;
-  return code;
-}
-
-int hipfail(atm_ctx* ctx, hipError_t e, const char* what) {
-  return fail(ctx, ATM_ERR_HIP, std::string(what) + ": " + hipGetErrorString(e));
-}
-
-#define HIPCHK(ctx, call)                               \
-  do {                                                  \
-    hipError_t _e = (call);                             \
-    if (_e != hipSuccess) return hipfail(ctx, _e, #call); \
-  } while (0)
 
 constexpr int EW_THREADS = 256;
 
@@ -192,57 +156,107 @@ __global__ void band_scale_kernel(BandGeo g, double* __restrict__ band, double s
   }
 }
 
-// 64-bit FNV-1a hash of the bit patterns of one line's band.
-__global__ void band_hash_kernel(BandGeo g, const double* __restrict__ band, uint64_t* __restrict__ hash) {
+// Line hashes for finding lines with bit-identical bands: 64-bit FNV-1a over
+// the bit patterns of each chunk of HASH_CHUNK band entries (one thread per
+// line and chunk), then FNV-1a over a line's chunk hashes in chunk order.
+constexpr int HASH_CHUNK = 64;
+
+__global__ void band_hash_chunk_kernel(BandGeo g, const double* __restrict__ band, uint64_t* __restrict__ part) {
+  const int64_t line = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (line >= g.nlines) return;
+  const int64_t m = (int64_t)g.n * g.nb;
+  const int64_t k0 = (int64_t)blockIdx.y * HASH_CHUNK;
+  const int64_t k1 = k0 + HASH_CHUNK < m ? k0 + HASH_CHUNK : m;
+  uint64_t h = 1469598103934665603ull;
+  for (int64_t k = k0; k < k1; ++k) {
+    h ^= (uint64_t)__double_as_longlong(band[k * g.nlines + line]);
+    h *= 1099511628211ull;
+  }
+  part[(int64_t)blockIdx.y * g.nlines + line] = h;
+}
+
+__global__ void band_hash_combine_kernel(BandGeo g, const uint64_t* __restrict__ part, int nchunk,
+                                         uint64_t* __restrict__ hash) {
   const int64_t line = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   if (line >= g.nlines) return;
   uint64_t h = 1469598103934665603ull;
-  const int64_t m = (int64_t)g.n * g.nb;
-  for (int64_t k = 0; k < m; ++k) {
-    h ^= (uint64_t)__double_as_longlong(band[k * g.nlines + line]);
+  for (int c = 0; c < nchunk; ++c) {
+    h ^= part[(int64_t)c * g.nlines + line];
     h *= 1099511628211ull;
   }
   hash[line] = h;
 }
 
-// same[line] = 1 if the line's band equals the band of rep[line] bit for bit.
-__global__ void band_verify_kernel(BandGeo g, const double* __restrict__ band, const int64_t* __restrict__ rep,
-                                   int32_t* __restrict__ same) {
+// same[line] is set to 1 beforehand; a chunk that differs bit-wise from the
+// same chunk of the representative line rep[line] clears it.
+__global__ void band_verify_chunk_kernel(BandGeo g, const double* __restrict__ band, const int64_t* __restrict__ rep,
+                                         int32_t* __restrict__ same) {
   const int64_t line = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   if (line >= g.nlines) return;
   const int64_t o = rep[line];
   const int64_t m = (int64_t)g.n * g.nb;
-  int32_t eq = 1;
-  for (int64_t k = 0; k < m && eq; ++k)
-    eq = __double_as_longlong(band[k * g.nlines + line]) == __double_as_longlong(band[k * g.nlines + o]);
-  same[line] = eq;
+  const int64_t k0 = (int64_t)blockIdx.y * HASH_CHUNK;
+  const int64_t k1 = k0 + HASH_CHUNK < m ? k0 + HASH_CHUNK : m;
+  bool eq = true;
+  for (int64_t k = k0; k < k1; ++k)
+    eq = eq && __double_as_longlong(band[k * g.nlines + line]) == __double_as_longlong(band[k * g.nlines + o]);
+  if (!eq) same[line] = 0;
 }
 
-// fac[k * nu + u] = band[k * nlines + src[u]]
+__global__ void fill_i32_kernel(int32_t* __restrict__ y, int32_t v, int64_t n) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    y[i] = v;
+}
+
+// fac[u * n*nb + k] = band[k * nlines + src[u]]
 __global__ void band_compact_kernel(BandGeo g, const double* __restrict__ band, const int64_t* __restrict__ src,
                                     int64_t nu, double* __restrict__ fac) {
-  const int64_t tot = (int64_t)g.n * g.nb * nu;
+  const int64_t M = (int64_t)g.n * g.nb;
+  const int64_t tot = M * nu;
   for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < tot; t += (int64_t)gridDim.x * blockDim.x) {
-    const int64_t k = t / nu, u = t - k * nu;
+    const int64_t u = t / M, k = t - u * M;
     fac[t] = band[k * g.nlines + src[u]];
   }
 }
 
-// In-place LU without pivoting (unit lower factor), one thread per distinct matrix.
-__global__ void band_lu_kernel(BandGeo g, double* __restrict__ fac, int64_t nu) {
-  const int64_t u = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-  if (u >= nu) return;
+// In-place LU without pivoting (unit lower factor), one workgroup per distinct
+// matrix: at step k the threads form the multipliers of rows k+1..k+bw, then
+// update the trailing (bw x bw) block.  Every entry sees the same operations
+// in the same order as in the sequential elimination.  With LDS_COPY the
+// matrix is staged in LDS ([r*nb + j]) for the elimination.
+constexpr int LU_THREADS = 256;
+
+template <bool LDS_COPY>
+__global__ void __launch_bounds__(LU_THREADS) band_lu_wg_kernel(BandGeo g, double* __restrict__ fac, int64_t nu) {
+  extern __shared__ double sA[];
+  const int64_t u = blockIdx.x;
   const int n = g.n, bw = g.bw, nb = g.nb;
-  auto A = [&](int r, int c) -> double& { return fac[((int64_t)r * nb + (bw + c - r)) * nu + u]; };
+  const int t = threadIdx.x;
+  const int64_t m = (int64_t)n * nb;
+  double* Fu = fac + u * m;
+  if (LDS_COPY)
+    for (int64_t k = t; k < m; k += LU_THREADS) sA[k] = Fu[k];
+  __syncthreads();
+  auto A = [&](int r, int c) -> double& {
+    const int64_t k = (int64_t)r * nb + (bw + c - r);
+    return LDS_COPY ? sA[k] : Fu[k];
+  };
   for (int k = 0; k < n; ++k) {
-    const double piv = A(k, k);
     const int iend = k + bw < n - 1 ? k + bw : n - 1;
-    for (int i = k + 1; i <= iend; ++i) {
-      const double l = A(i, k) / piv;
-      A(i, k) = l;
-      for (int j = k + 1; j <= iend; ++j) A(i, j) -= l * A(k, j);
+    const int w = iend - k;
+    if (t < w) {
+      const int i = k + 1 + t;
+      A(i, k) = A(i, k) / A(k, k);
     }
+    __syncthreads();
+    for (int q = t; q < w * w; q += LU_THREADS) {
+      const int i = k + 1 + q / w, j = k + 1 + q % w;
+      A(i, j) -= A(i, k) * A(k, j);
+    }
+    __syncthreads();
   }
+  if (LDS_COPY)
+    for (int64_t k = t; k < m; k += LU_THREADS) Fu[k] = sA[k];
 }
 
 // Forward and back substitution, one thread (lane) per vertical line; the last
@@ -271,8 +285,8 @@ band_solve_kernel(BandGeo g, const double* __restrict__ fac, const int32_t* __re
         const int64_t idx = (int64_t)s * g.Ntot + base;
         double v = b[idx];
         const int lo = r - bw > 0 ? r - bw : 0;
-        const double* arow = fac + ((int64_t)r * nb + bw - r) * nu + mat;
-        for (int cc = lo; cc < r; ++cc) v -= arow[(int64_t)cc * nu] * win[cc & (BAND_WIN - 1)][lane];
+        const double* arow = fac + mat * ((int64_t)n * nb) + (int64_t)r * nb + bw - r;
+        for (int cc = lo; cc < r; ++cc) v -= arow[cc] * win[cc & (BAND_WIN - 1)][lane];
         win[r & (BAND_WIN - 1)][lane] = v;
         x[idx] = v;
       }
@@ -287,15 +301,234 @@ band_solve_kernel(BandGeo g, const double* __restrict__ fac, const int32_t* __re
         const int64_t idx = (int64_t)s * g.Ntot + base;
         double v = x[idx];
         const int hi = r + bw < n - 1 ? r + bw : n - 1;
-        const double* arow = fac + ((int64_t)r * nb + bw - r) * nu + mat;
-        for (int cc = r + 1; cc <= hi; ++cc) v -= arow[(int64_t)cc * nu] * win[cc & (BAND_WIN - 1)][lane];
-        v = v / arow[(int64_t)r * nu];
+        const double* arow = fac + mat * ((int64_t)n * nb) + (int64_t)r * nb + bw - r;
+        for (int cc = r + 1; cc <= hi; ++cc) v -= arow[cc] * win[cc & (BAND_WIN - 1)][lane];
+        v = v / arow[r];
         win[r & (BAND_WIN - 1)][lane] = v;
         x[idx] = v;
       }
     }
   }
 }
+
+// Forward and back substitution with the line vectors staged in LDS: a
+// workgroup (one wave) takes 64/PARTS lines, PARTS lanes per line; each lane
+// sums every PARTS-th product of a row's band window and the lanes of a line
+// combine their partial sums by cross-lane exchange.  Loads and stores of the
+// line vectors are coalesced over the lines' horizontal node index.
+// LDS: y[r * (LINES + 1) + j].  x may alias b (all of a line is read first).
+template <int PARTS>
+__global__ void __launch_bounds__(64)
+band_solve_lds_kernel(BandGeo g, const double* __restrict__ fac, const int32_t* __restrict__ line_mat, int64_t nu,
+                      double* x, const double* b) {
+  constexpr int LINES = 64 / PARTS, LD = LINES + 1;
+  extern __shared__ double y[];
+  const int t = threadIdx.x, j = t / PARTS, part = t % PARTS;
+  const int64_t line = blockIdx.x * (int64_t)LINES + j;
+  const bool active = line < g.nlines;
+  const int n = g.n, bw = g.bw, nb = g.nb;
+  const int64_t c = active ? line / g.Nh : 0;
+  const int h = active ? (int)(line - c * g.Nh) : 0;
+  const int32_t* col = g.colelem + c * g.KV;
+  const int64_t mat = active ? line_mat[line] : 0;
+  // unknown r = s + Ns*(k + Nq*ez) lives at s*Ntot + h + Nh*k + Np*col[ez]
+  auto gidx = [&](int r) -> int64_t {
+    const int s = r % g.Ns, rem = r / g.Ns;
+    const int k = rem % g.Nq, ez = rem / g.Nq;
+    return (int64_t)s * g.Ntot + h + (int64_t)g.Nh * k + (int64_t)g.Np * col[ez];
+  };
+  for (int r = part; r < n; r += PARTS) y[r * LD + j] = active ? b[gidx(r)] : 0.0;
+  __syncthreads();
+  for (int r = 0; r < n; ++r) {
+    const int lo = r - bw > 0 ? r - bw : 0;
+    const double* arow = fac + mat * ((int64_t)n * nb) + (int64_t)r * nb + bw - r;
+    double acc = 0.0;
+    for (int cc = lo + part; cc < r; cc += PARTS) acc += arow[cc] * y[cc * LD + j];
+#pragma unroll
+    for (int o = 1; o < PARTS; o <<= 1) acc += __shfl_xor(acc, o, 64);
+    const double v = y[r * LD + j] - acc;
+    __syncthreads();
+    if (part == 0) y[r * LD + j] = v;
+    __syncthreads();
+  }
+  for (int r = n - 1; r >= 0; --r) {
+    const int hi = r + bw < n - 1 ? r + bw : n - 1;
+    const double* arow = fac + mat * ((int64_t)n * nb) + (int64_t)r * nb + bw - r;
+    double acc = 0.0;
+    for (int cc = r + 1 + part; cc <= hi; cc += PARTS) acc += arow[cc] * y[cc * LD + j];
+#pragma unroll
+    for (int o = 1; o < PARTS; o <<= 1) acc += __shfl_xor(acc, o, 64);
+    const double v = (y[r * LD + j] - acc) / arow[r];
+    __syncthreads();
+    if (part == 0) y[r * LD + j] = v;
+    __syncthreads();
+  }
+  if (!active) return;
+  for (int r = part; r < n; r += PARTS) x[gidx(r)] = y[r * LD + j];
+}
+
+// Forward and back substitution with one lane per line and the unknowns of
+// one element of the line (BS = NS*NQ rows, the band half-width + 1) held in
+// registers, fully unrolled: row r depends on the previous block's and the
+// current block's values only, so the products of a row are independent
+// except for the last few, and the rows of a block overlap (each block of rows
+// is one branch-free sequence; the first and last blocks are separate
+// instantiations).  Lines are taken in the order perm (grouped by matrix, each
+// group padded to 64 with -1), so every wave solves with one matrix,
+// wave_mat[wave].  The wave loads the next block's factor rows (BS x nb)
+// cooperatively while it computes the current block and stages them in LDS,
+// where every lane reads them by broadcast; the next block's right-hand side
+// is prefetched into registers.  Same operations in the same order as
+// band_solve_kernel.
+
+// rows of block ez (FIRST: ez == 0, no previous block): y = b - L y
+template <int BS, bool FIRST>
+__device__ __forceinline__ void band_fwd_rows(const double* __restrict__ blk, double* cur, const double* prev) {
+  constexpr int bw = BS - 1, nb = 2 * bw + 1;
+#pragma unroll
+  for (int m = 0; m < BS; ++m) {
+    double v = cur[m];
+    if constexpr (!FIRST) {
+#pragma unroll
+      for (int p = m + 1; p < BS; ++p) v -= blk[m * nb + bw + p - m - BS] * prev[p];
+    }
+#pragma unroll
+    for (int q = 0; q < m; ++q) v -= blk[m * nb + bw + q - m] * cur[q];
+    cur[m] = v;
+  }
+}
+
+// rows of block ez backwards (LAST: ez == KV-1, no next block): x = (y - U x) / U_rr
+template <int BS, bool LAST>
+__device__ __forceinline__ void band_bwd_rows(const double* __restrict__ blk, double* cur, const double* next) {
+  constexpr int bw = BS - 1, nb = 2 * bw + 1;
+#pragma unroll
+  for (int m = BS - 1; m >= 0; --m) {
+    double v = cur[m];
+#pragma unroll
+    for (int q = m + 1; q < BS; ++q) v -= blk[m * nb + bw + q - m] * cur[q];
+    if constexpr (!LAST) {
+#pragma unroll
+      for (int p = 0; p < m; ++p) v -= blk[m * nb + bw + BS + p - m] * next[p];
+    }
+    cur[m] = v / blk[m * nb + bw];
+  }
+}
+
+template <int NS, int NQ>
+__global__ void __launch_bounds__(64)
+band_solve_reg_kernel(BandGeo g, const double* __restrict__ fac, const int32_t* __restrict__ perm,
+                      const int32_t* __restrict__ wave_mat, double* x, const double* b) {
+  constexpr int BS = NS * NQ, bw = BS - 1, nb = 2 * bw + 1;
+  constexpr int FB = BS * nb;          // factor entries of one block of rows
+  constexpr int FPL = (FB + 63) / 64;  // per lane
+  __shared__ double fbuf[2][FB];
+  const int lane = threadIdx.x;
+  const int32_t pl = perm[blockIdx.x * (int64_t)64 + lane];
+  const bool active = pl >= 0;
+  const int64_t line = active ? pl : 0;  // padding lanes read line 0 and store nothing
+  const double* F = fac + (int64_t)wave_mat[blockIdx.x] * ((int64_t)g.n * nb);
+  const int KV = g.KV;
+  const int64_t c = line / g.Nh;
+  const int h = (int)(line - c * g.Nh);
+  const int32_t* col = g.colelem + c * KV;
+  auto gi = [&](int ez, int m) -> int64_t {
+    return (int64_t)(m % NS) * g.Ntot + h + (int64_t)g.Np * col[ez] + (int64_t)g.Nh * (m / NS);
+  };
+  double pf[FPL];
+  auto fetch = [&](int ez) {
+#pragma unroll
+    for (int i = 0; i < FPL; ++i) {
+      const int k = lane + 64 * i;
+      if (k < FB) pf[i] = F[(int64_t)ez * FB + k];
+    }
+  };
+  auto stage = [&](int buf) {
+#pragma unroll
+    for (int i = 0; i < FPL; ++i) {
+      const int k = lane + 64 * i;
+      if (k < FB) fbuf[buf][k] = pf[i];
+    }
+  };
+  auto load = [&](const double* src, int ez, double* dst) {
+#pragma unroll
+    for (int m = 0; m < BS; ++m) dst[m] = src[gi(ez, m)];
+  };
+  auto store = [&](int ez, const double* v) {
+    if (active) {
+#pragma unroll
+      for (int m = 0; m < BS; ++m) x[gi(ez, m)] = v[m];
+    }
+  };
+  double prev[BS], cur[BS], nxt[BS];
+  // ---- forward: L y = b (y stored in x)
+  fetch(0);
+  stage(0);
+  load(b, 0, nxt);
+  __syncthreads();
+  for (int ez = 0; ez < KV; ++ez) {
+    const int buf = ez & 1;
+#pragma unroll
+    for (int m = 0; m < BS; ++m) cur[m] = nxt[m];
+    if (ez + 1 < KV) {
+      fetch(ez + 1);
+      load(b, ez + 1, nxt);
+    }
+    if (ez == 0) band_fwd_rows<BS, true>(fbuf[buf], cur, prev);
+    else band_fwd_rows<BS, false>(fbuf[buf], cur, prev);
+    store(ez, cur);
+#pragma unroll
+    for (int m = 0; m < BS; ++m) prev[m] = cur[m];
+    if (ez + 1 < KV) stage(1 - buf);
+    __syncthreads();
+  }
+  // ---- backward: U x = y
+  fetch(KV - 1);
+  stage(0);
+  load(x, KV - 1, nxt);
+  __syncthreads();
+  for (int ez = KV - 1, it = 0; ez >= 0; --ez, ++it) {
+    const int buf = it & 1;
+#pragma unroll
+    for (int m = 0; m < BS; ++m) cur[m] = nxt[m];
+    if (ez > 0) {
+      fetch(ez - 1);
+      load(x, ez - 1, nxt);
+    }
+    if (ez == KV - 1) band_bwd_rows<BS, true>(fbuf[buf], cur, prev);
+    else band_bwd_rows<BS, false>(fbuf[buf], cur, prev);
+    store(ez, cur);
+#pragma unroll
+    for (int m = 0; m < BS; ++m) prev[m] = cur[m];
+    if (ez > 0) stage(1 - buf);
+    __syncthreads();
+  }
+}
+
+typedef void (*BandSolveRegFn)(BandGeo, const double*, const int32_t*, const int32_t*, double*, const double*);
+
+// register-blocked kernels for the state counts of the vertical linear laws
+// (advection 1, linear Euler-gravity 4 and 5) and NQ = 2..6
+BandSolveRegFn band_reg_kernel(int Ns, int Nq) {
+#define ATM_BAND_NQ(NS)                                \
+  switch (Nq) {                                        \
+    case 2: return band_solve_reg_kernel<NS, 2>;       \
+    case 3: return band_solve_reg_kernel<NS, 3>;       \
+    case 4: return band_solve_reg_kernel<NS, 4>;       \
+    case 5: return band_solve_reg_kernel<NS, 5>;       \
+    case 6: return band_solve_reg_kernel<NS, 6>;       \
+    default: return nullptr;                           \
+  }
+  if (Ns == 1) ATM_BAND_NQ(1)
+  if (Ns == 4) ATM_BAND_NQ(4)
+  if (Ns == 5) ATM_BAND_NQ(5)
+#undef ATM_BAND_NQ
+  return nullptr;
+}
+
+constexpr int BAND_SOLVE_PARTS = 4;
+
+size_t band_solve_lds_bytes(int n) { return (size_t)n * (64 / BAND_SOLVE_PARTS + 1) * sizeof(double); }
 
 BandGeo band_geo(const atm_banded* b) {
   BandGeo g;
@@ -385,18 +618,6 @@ bool sflux_ok(int law, int dim, int sf) {
   }
 }
 
-template <class T>
-int upload_copy(atm_dg* dg, const T* host, size_t count, const T** out) {
-  *out = nullptr;
-  if (count == 0) return ATM_OK;
-  if (!host) return fail(dg->ctx, ATM_ERR_ARG, "atm_dg_create: missing geometry array");
-  void* p = nullptr;
-  HIPCHK(dg->ctx, hipMalloc(&p, count * sizeof(T)));
-  dg->allocs.push_back(p);
-  HIPCHK(dg->ctx, hipMemcpy(p, host, count * sizeof(T), hipMemcpyHostToDevice));
-  *out = static_cast<const T*>(p);
-  return ATM_OK;
-}
 
 // reduction buffer of an operator: partials, then RED_STATE relaxation slots, then RED_OUT results
 constexpr int64_t RED_PART = (int64_t)RED_MAXBLK * RED_MAXQ * 2;
@@ -466,8 +687,14 @@ int atm_ctx_destroy(atm_ctx* ctx) {
   if (!ctx) return ATM_ERR_ARG;
   hipSetDevice(ctx->device);
   hipStreamSynchronize(ctx->stream);
+  if (ctx->copy) hipStreamSynchronize(ctx->copy);
   for (hipEvent_t ev : ctx->events)
     if (ev) hipEventDestroy(ev);
+  for (int i = 0; i < 16; ++i) {
+    if (ctx->snap_ready[i]) hipEventDestroy(ctx->snap_ready[i]);
+    if (ctx->snap_done[i]) hipEventDestroy(ctx->snap_done[i]);
+  }
+  if (ctx->copy) hipStreamDestroy(ctx->copy);
   if (ctx->own) hipStreamDestroy(ctx->own);
   delete ctx;
   return ATM_OK;
@@ -524,7 +751,18 @@ int atm_copy(atm_ctx* ctx, void* dst, const void* src, size_t bytes) {
   return ATM_OK;
 }
 
-int atm_dg_create(atm_ctx* ctx, const atm_dg_desc* d, atm_dg** out) {
+}  // extern "C"
+
+namespace atm {
+
+int dg_alloc(atm_dg* dg, size_t bytes, void** out) {
+  *out = nullptr;
+  HIPCHK(dg->ctx, hipMalloc(out, bytes));
+  dg->allocs.push_back(*out);
+  return ATM_OK;
+}
+
+int dg_prepare(atm_ctx* ctx, const atm_dg_desc* d, atm_dg** out) {
   if (!ctx || !d || !out) return fail(ctx, ATM_ERR_ARG, "atm_dg_create: null argument");
   *out = nullptr;
   if (d->dim < 1 || d->dim > 3) return fail(ctx, ATM_ERR_ARG, "dim must be 1, 2 or 3");
@@ -548,38 +786,12 @@ int atm_dg_create(atm_ctx* ctx, const atm_dg_desc* d, atm_dg** out) {
       (d->law == ATM_LAW_LINEAR_EULER_GRAVITY && d->naux < 3 + d->dim))
     return fail(ctx, ATM_ERR_ARG, "aux state has too few components for the law");
   if (d->naux > 3 + d->dim) return fail(ctx, ATM_ERR_ARG, "aux state has too many components");
+  if (d->naux > 0 && d->K > 0 && !d->aux) return fail(ctx, ATM_ERR_ARG, "missing aux array");
   if (d->bc == ATM_BC_PRESCRIBED && (d->n_bc_states < 1 || !d->bc_states))
     return fail(ctx, ATM_ERR_ARG, "prescribed boundary needs bc_states");
   if (r.lds > ctx->max_lds) return fail(ctx, ATM_ERR_UNSUPPORTED, "element does not fit in LDS");
   const int Nfpf = Np / Nq, Nfp = 2 * d->dim * Nfpf;
   const int64_t Ntot = (int64_t)Np * d->K, NF = (int64_t)Nfp * d->K;
-  // Face points must follow the canonical order: face 2r+side, other node
-  // indices ascending with the lower dimension fastest.
-  if (d->K > 0 && !d->faceix_m) return fail(ctx, ATM_ERR_ARG, "missing faceix_m");
-  for (int64_t e = 0; e < d->K; ++e)
-    for (int face = 0; face < 2 * d->dim; ++face) {
-      const int rr = face / 2, side = face % 2;
-      for (int fp = 0; fp < Nfpf; ++fp) {
-        int id[3] = {0, 0, 0}, rem = fp;
-        for (int dd = 0; dd < d->dim; ++dd) {
-          if (dd == rr) continue;
-          id[dd] = rem % Nq;
-          rem /= Nq;
-        }
-        id[rr] = side ? Nq - 1 : 0;
-        const int64_t want = id[0] + Nq * (id[1] + (int64_t)Nq * id[2]) + (int64_t)Np * e;
-        if (d->faceix_m[e * Nfp + face * Nfpf + fp] != want)
-          return fail(ctx, ATM_ERR_UNSUPPORTED, "faceix_m is not in canonical face-point order");
-      }
-    }
-  // every index the kernels follow must stay inside the node array / state table
-  if (d->K > 0 && (!d->faceix_p || !d->bctag || !d->MJ || !d->MJI || !d->g || !d->D || !d->faceMJ || !d->normal))
-    return fail(ctx, ATM_ERR_ARG, "missing geometry array");
-  if (d->naux > 0 && !d->aux) return fail(ctx, ATM_ERR_ARG, "missing aux array");
-  for (int64_t i = 0; i < NF; ++i)
-    if (d->faceix_p[i] < 0 || d->faceix_p[i] >= Ntot) return fail(ctx, ATM_ERR_ARG, "faceix_p out of range");
-  for (int64_t i = 0; i < (int64_t)2 * d->dim * d->K; ++i)
-    if (d->bctag[i] < 0) return fail(ctx, ATM_ERR_ARG, "negative boundary tag");
   hipSetDevice(ctx->device);
   atm_dg* dg = new atm_dg();
   dg->ctx = ctx;
@@ -615,6 +827,61 @@ int atm_dg_create(atm_ctx* ctx, const atm_dg_desc* d, atm_dg** out) {
   p.Ntot = Ntot;
   p.NF = NF;
   int rc = ATM_OK;
+  if ((rc = upload_copy(dg, d->aux, (size_t)d->naux * Ntot, &p.aux)) ||
+      (rc = upload_copy(dg, d->bc_states, (size_t)(d->bc == ATM_BC_PRESCRIBED ? d->n_bc_states * r.ns : 0),
+                        &p.c.bc_states))) {
+    atm_dg_destroy(dg);
+    return rc;
+  }
+  *out = dg;
+  return ATM_OK;
+}
+
+}  // namespace atm
+
+extern "C" {
+
+int atm_dg_create(atm_ctx* ctx, const atm_dg_desc* d, atm_dg** out) {
+  if (!ctx || !d || !out) return fail(ctx, ATM_ERR_ARG, "atm_dg_create: null argument");
+  *out = nullptr;
+  if (d->dim < 1 || d->dim > 3) return fail(ctx, ATM_ERR_ARG, "dim must be 1, 2 or 3");
+  if (d->N < 1 || d->N > 7) return fail(ctx, ATM_ERR_UNSUPPORTED, "polynomial degree must be 1..7");
+  if (d->K < 0) return fail(ctx, ATM_ERR_ARG, "K must be >= 0");
+  const int Nq = d->N + 1;
+  int Np = 1;
+  for (int i = 0; i < d->dim; ++i) Np *= Nq;
+  const int Nfpf = Np / Nq, Nfp = 2 * d->dim * Nfpf;
+  const int64_t Ntot = (int64_t)Np * d->K, NF = (int64_t)Nfp * d->K;
+  // Face points must follow the canonical order: face 2r+side, other node
+  // indices ascending with the lower dimension fastest.
+  if (d->K > 0 && !d->faceix_m) return fail(ctx, ATM_ERR_ARG, "missing faceix_m");
+  for (int64_t e = 0; e < d->K; ++e)
+    for (int face = 0; face < 2 * d->dim; ++face) {
+      const int rr = face / 2, side = face % 2;
+      for (int fp = 0; fp < Nfpf; ++fp) {
+        int id[3] = {0, 0, 0}, rem = fp;
+        for (int dd = 0; dd < d->dim; ++dd) {
+          if (dd == rr) continue;
+          id[dd] = rem % Nq;
+          rem /= Nq;
+        }
+        id[rr] = side ? Nq - 1 : 0;
+        const int64_t want = id[0] + Nq * (id[1] + (int64_t)Nq * id[2]) + (int64_t)Np * e;
+        if (d->faceix_m[e * Nfp + face * Nfpf + fp] != want)
+          return fail(ctx, ATM_ERR_UNSUPPORTED, "faceix_m is not in canonical face-point order");
+      }
+    }
+  // every index the kernels follow must stay inside the node array / state table
+  if (d->K > 0 && (!d->faceix_p || !d->bctag || !d->MJ || !d->MJI || !d->g || !d->D || !d->faceMJ || !d->normal))
+    return fail(ctx, ATM_ERR_ARG, "missing geometry array");
+  for (int64_t i = 0; i < NF; ++i)
+    if (d->faceix_p[i] < 0 || d->faceix_p[i] >= Ntot) return fail(ctx, ATM_ERR_ARG, "faceix_p out of range");
+  for (int64_t i = 0; i < (int64_t)2 * d->dim * d->K; ++i)
+    if (d->bctag[i] < 0) return fail(ctx, ATM_ERR_ARG, "negative boundary tag");
+  atm_dg* dg = nullptr;
+  int rc = dg_prepare(ctx, d, &dg);
+  if (rc) return rc;
+  DevDG& p = dg->dev;
   const size_t dd = (size_t)d->dim * d->dim;
   if ((rc = upload_copy(dg, d->D, (size_t)Nq * Nq, &p.D)) ||
       (rc = upload_copy(dg, d->g, dd * Ntot, &p.g)) ||
@@ -622,11 +889,8 @@ int atm_dg_create(atm_ctx* ctx, const atm_dg_desc* d, atm_dg** out) {
       (rc = upload_copy(dg, d->MJI, (size_t)Ntot, &p.MJI)) ||
       (rc = upload_copy(dg, d->faceMJ, (size_t)NF, &p.faceMJ)) ||
       (rc = upload_copy(dg, d->normal, (size_t)d->dim * NF, &p.normal)) ||
-      (rc = upload_copy(dg, d->aux, (size_t)d->naux * Ntot, &p.aux)) ||
       (rc = upload_copy(dg, d->faceix_p, (size_t)NF, &p.faceix_p)) ||
-      (rc = upload_copy(dg, d->bctag, (size_t)2 * d->dim * d->K, &p.bctag)) ||
-      (rc = upload_copy(dg, d->bc_states, (size_t)(d->bc == ATM_BC_PRESCRIBED ? d->n_bc_states * r.ns : 0),
-                        &p.c.bc_states))) {
+      (rc = upload_copy(dg, d->bctag, (size_t)2 * d->dim * d->K, &p.bctag))) {
     atm_dg_destroy(dg);
     return rc;
   }
@@ -1077,10 +1341,20 @@ int atm_banded_create(atm_dg* lin, int32_t KV, const int32_t* colelem, int32_t n
   if ((e = hipMalloc(&drep, L * sizeof(int64_t) + 8)) != hipSuccess) return done(hipfail(ctx, e, "hipMalloc"));
   if ((e = hipMalloc(&dsame, L * sizeof(int32_t) + 8)) != hipSuccess) return done(hipfail(ctx, e, "hipMalloc"));
   const int64_t lblocks = (L + 63) / 64;
-  if (lblocks > 0) hipLaunchKernelGGL(band_hash_kernel, dim3(lblocks), dim3(64), 0, s, g, band, dhash);
+  const int nchunk = (int)(((int64_t)b->n * b->nb + HASH_CHUNK - 1) / HASH_CHUNK);
+  uint64_t* dpart = nullptr;
+  if ((e = hipMalloc(&dpart, (size_t)nchunk * L * sizeof(uint64_t) + 8)) != hipSuccess)
+    return done(hipfail(ctx, e, "hipMalloc"));
+  if (lblocks > 0) {
+    hipLaunchKernelGGL(band_hash_chunk_kernel, dim3(lblocks, nchunk), dim3(64), 0, s, g, band, dpart);
+    hipLaunchKernelGGL(band_hash_combine_kernel, dim3(lblocks), dim3(64), 0, s, g, dpart, nchunk, dhash);
+  }
   if ((e = hipMemcpyAsync(hash.data(), dhash, L * sizeof(uint64_t), hipMemcpyDeviceToHost, s)) != hipSuccess ||
-      (e = hipStreamSynchronize(s)) != hipSuccess)
+      (e = hipStreamSynchronize(s)) != hipSuccess) {
+    hipFree(dpart);
     return done(hipfail(ctx, e, "banded assembly"));
+  }
+  hipFree(dpart);
   {
     std::vector<std::pair<uint64_t, int64_t>> first;
     first.reserve(64);
@@ -1097,7 +1371,10 @@ int atm_banded_create(atm_dg* lin, int32_t KV, const int32_t* colelem, int32_t n
   }
   if ((e = hipMemcpyAsync(drep, rep.data(), L * sizeof(int64_t), hipMemcpyHostToDevice, s)) != hipSuccess)
     return done(hipfail(ctx, e, "hipMemcpy"));
-  if (lblocks > 0) hipLaunchKernelGGL(band_verify_kernel, dim3(lblocks), dim3(64), 0, s, g, band, drep, dsame);
+  if (lblocks > 0) {
+    hipLaunchKernelGGL(fill_i32_kernel, dim3(lblocks), dim3(64), 0, s, dsame, 1, L);
+    hipLaunchKernelGGL(band_verify_chunk_kernel, dim3(lblocks, nchunk), dim3(64), 0, s, g, band, drep, dsame);
+  }
   if ((e = hipMemcpyAsync(same.data(), dsame, L * sizeof(int32_t), hipMemcpyDeviceToHost, s)) != hipSuccess ||
       (e = hipStreamSynchronize(s)) != hipSuccess)
     return done(hipfail(ctx, e, "banded verification"));
@@ -1113,6 +1390,25 @@ int atm_banded_create(atm_dg* lin, int32_t KV, const int32_t* colelem, int32_t n
     b->line_mat_host[l] = (int32_t)uid[r0];
   }
   b->nu = (int64_t)src.size();
+  // lines grouped by matrix (ascending line order within a group), each group padded to whole waves
+  std::vector<int32_t> perm, wmat;
+  {
+    std::vector<std::vector<int32_t>> by(b->nu);
+    for (int64_t l = 0; l < L; ++l) by[b->line_mat_host[l]].push_back((int32_t)l);
+    for (int64_t u = 0; u < b->nu; ++u) {
+      for (size_t i = 0; i < by[u].size(); i += 64) {
+        for (size_t k = i; k < i + 64; ++k) perm.push_back(k < by[u].size() ? by[u][k] : -1);
+        wmat.push_back((int32_t)u);
+      }
+    }
+  }
+  b->nwaves = (int64_t)wmat.size();
+  if ((e = hipMalloc(&b->perm, perm.size() * sizeof(int32_t) + 8)) != hipSuccess ||
+      (e = hipMalloc(&b->wave_mat, wmat.size() * sizeof(int32_t) + 8)) != hipSuccess)
+    return done(hipfail(ctx, e, "hipMalloc"));
+  if (!perm.empty() && ((e = hipMemcpy(b->perm, perm.data(), perm.size() * sizeof(int32_t), hipMemcpyHostToDevice)) != hipSuccess ||
+                        (e = hipMemcpy(b->wave_mat, wmat.data(), wmat.size() * sizeof(int32_t), hipMemcpyHostToDevice)) != hipSuccess))
+    return done(hipfail(ctx, e, "hipMemcpy"));
   const size_t facsz = (size_t)b->n * b->nb * b->nu;
   if ((e = hipMalloc(&b->fac, facsz * sizeof(double) + 8)) != hipSuccess) return done(hipfail(ctx, e, "hipMalloc factors"));
   if ((e = hipMalloc(&b->line_mat, L * sizeof(int32_t) + 8)) != hipSuccess) return done(hipfail(ctx, e, "hipMalloc"));
@@ -1123,8 +1419,14 @@ int atm_banded_create(atm_dg* lin, int32_t KV, const int32_t* colelem, int32_t n
   if (facsz > 0)
     hipLaunchKernelGGL(band_compact_kernel, dim3(ew_blocks(facsz)), dim3(EW_THREADS), 0, s, g, band, drep, b->nu,
                        b->fac);
-  const int64_t ublocks = (b->nu + 63) / 64;
-  if (ublocks > 0) hipLaunchKernelGGL(band_lu_kernel, dim3(ublocks), dim3(64), 0, s, g, b->fac, b->nu);
+  if (b->nu > 0) {
+    const size_t lu_lds = (size_t)b->n * b->nb * sizeof(double);
+    if (lu_lds <= ctx->max_lds)
+      hipLaunchKernelGGL(band_lu_wg_kernel<true>, dim3(b->nu), dim3(LU_THREADS), lu_lds, s, g, b->fac, b->nu);
+    else
+      hipLaunchKernelGGL(band_lu_wg_kernel<false>, dim3(b->nu), dim3(LU_THREADS), 0, s, g, b->fac, b->nu);
+    if ((e = hipGetLastError()) != hipSuccess) return done(hipfail(ctx, e, "band LU launch"));
+  }
   if ((e = hipStreamSynchronize(s)) != hipSuccess) return done(hipfail(ctx, e, "banded factorisation"));
   rc = done(ATM_OK);
   if (rc) return rc;
@@ -1136,6 +1438,8 @@ int atm_banded_destroy(atm_banded* b) {
   if (!b) return ATM_ERR_ARG;
   if (b->fac) hipFree(b->fac);
   if (b->line_mat) hipFree(b->line_mat);
+  if (b->perm) hipFree(b->perm);
+  if (b->wave_mat) hipFree(b->wave_mat);
   if (b->elem_col) hipFree(b->elem_col);
   if (b->elem_ez) hipFree(b->elem_ez);
   if (b->colelem) hipFree(b->colelem);
@@ -1147,10 +1451,22 @@ int atm_banded_solve(atm_banded* b, double* x, const double* rhs) {
   if (!b || !x || !rhs) return ATM_ERR_ARG;
   atm_ctx* ctx = b->lin->ctx;
   const BandGeo g = band_geo(b);
-  const int64_t blocks = (b->nlines + 63) / 64;
-  if (blocks > 0)
-    hipLaunchKernelGGL(band_solve_kernel, dim3(blocks), dim3(64), 0, ctx->stream, g, b->fac, b->line_mat, b->nu, x,
-                       rhs);
+  const size_t lds = band_solve_lds_bytes(b->n);
+  if (BandSolveRegFn fn = band_reg_kernel(b->lin->Ns, b->lin->Nq)) {
+    if (b->nwaves > 0)
+      hipLaunchKernelGGL(fn, dim3(b->nwaves), dim3(64), 0, ctx->stream, g, b->fac, b->perm, b->wave_mat, x, rhs);
+  } else if (lds <= ctx->max_lds) {
+    constexpr int LINES = 64 / BAND_SOLVE_PARTS;
+    const int64_t blocks = (b->nlines + LINES - 1) / LINES;
+    if (blocks > 0)
+      hipLaunchKernelGGL(band_solve_lds_kernel<BAND_SOLVE_PARTS>, dim3(blocks), dim3(64), lds, ctx->stream, g,
+                         b->fac, b->line_mat, b->nu, x, rhs);
+  } else {
+    const int64_t blocks = (b->nlines + 63) / 64;
+    if (blocks > 0)
+      hipLaunchKernelGGL(band_solve_kernel, dim3(blocks), dim3(64), 0, ctx->stream, g, b->fac, b->line_mat, b->nu, x,
+                         rhs);
+  }
   HIPCHK(ctx, hipGetLastError());
   return ATM_OK;
 }
@@ -1177,7 +1493,7 @@ int atm_banded_download(atm_banded* b, double* host) {
     const int64_t u = b->line_mat_host[line];
     for (int r = 0; r < b->n; ++r)
       for (int k = 0; k < b->nb; ++k)
-        host[((size_t)line * b->n + r) * b->nb + k] = tmp[((size_t)r * b->nb + k) * b->nu + u];
+        host[((size_t)line * b->n + r) * b->nb + k] = tmp[((size_t)u * b->n + r) * b->nb + k];
   }
   return ATM_OK;
 }

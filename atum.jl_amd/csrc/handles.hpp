@@ -1,0 +1,82 @@
+// Library-internal definitions shared by the translation units of
+// libatumhip.so: the context and operator handles and the error helpers.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <string>
+#include <vector>
+
+#include "measures.hpp"
+#include "resolve.hpp"
+
+using namespace atm;
+
+struct atm_ctx {
+  int device = 0;
+  hipStream_t own = nullptr;
+  hipStream_t stream = nullptr;
+  size_t max_lds = 65536;
+  std::string err;
+  hipEvent_t events[16] = {};
+  // snapshots: a second stream (default priority) for device-to-host copies
+  hipStream_t copy = nullptr;
+  hipEvent_t snap_ready[16] = {}, snap_done[16] = {};
+};
+
+struct atm_dg {
+  atm_ctx* ctx = nullptr;
+  DevDG dev{};
+  LaunchFn fn = nullptr;
+  BatchLaunchFn bfn = nullptr;
+  size_t lds = 0;
+  int epw = 1;
+  int law = 0, dim = 0, Nq = 0, Np = 0, Ns = 0;
+  int64_t Ntot = 0;
+  MeasureFns mfn;
+  double* points = nullptr;  // node coordinates (dim blocks of Ntot) when the geometry was built on the device
+  double* red = nullptr;  // reduction partials, relaxation scalars, results (allocated on first use)
+  std::vector<void*> allocs;
+};
+
+
+namespace atm {
+
+inline int fail(atm_ctx* ctx, int code, const std::string& msg) {
+  if (ctx) ctx->err = msg;
+  return code;
+}
+
+inline int hipfail(atm_ctx* ctx, hipError_t e, const char* what) {
+  return fail(ctx, ATM_ERR_HIP, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+#define HIPCHK(ctx, call)                               \
+  do {                                                  \
+    hipError_t _e = (call);                             \
+    if (_e != hipSuccess) return hipfail(ctx, _e, #call); \
+  } while (0)
+
+
+// Validates everything in d except the geometry arrays, resolves the kernels
+// and creates the operator with its device description filled in (geometry,
+// aux and boundary-state pointers still null).
+int dg_prepare(atm_ctx* ctx, const atm_dg_desc* d, atm_dg** out);
+
+// Device allocation owned by the operator (freed by atm_dg_destroy).
+int dg_alloc(atm_dg* dg, size_t bytes, void** out);
+
+// Allocation owned by the operator, filled from host memory (count == 0: null).
+template <class T>
+int upload_copy(atm_dg* dg, const T* host, size_t count, const T** out) {
+  *out = nullptr;
+  if (count == 0) return ATM_OK;
+  if (!host) return fail(dg->ctx, ATM_ERR_ARG, "missing array");
+  void* p = nullptr;
+  int rc = dg_alloc(dg, count * sizeof(T), &p);
+  if (rc) return rc;
+  HIPCHK(dg->ctx, hipMemcpy(p, host, count * sizeof(T), hipMemcpyHostToDevice));
+  *out = static_cast<const T*>(p);
+  return ATM_OK;
+}
+
+}  // namespace atm

@@ -222,6 +222,57 @@ int atm_event_elapsed_ms(atm_ctx* ctx, int slot_start, int slot_end, float* ms);
 /* y = a*y + b*x over n doubles */
 int atm_axpby(atm_ctx* ctx, double* y, const double* x, double a, double b, int64_t n);
 
+/* ---- brick meshes built on the device ---------------------------------------
+ * Replaces brickgrid(LobattoCell, coordinates; periodic, ordering) (Bennu,
+ * called at test/euler/isentropicvortex.jl:43 and experiment/*) followed by
+ * DGSEM(...) (src/dgsem.jl:38-56): the library builds the node coordinates,
+ * metric terms, MJ, MJI, faceMJ, normals, face tables and boundary tags on the
+ * device, bit-identical to the host builder.  Vertex coordinates, LGL nodes and
+ * weights come from the host; in the operator description only D, aux and
+ * bc_states are read (the other geometry pointers are ignored). */
+typedef struct {
+  int32_t Kd[3];            /* elements per dimension (dim entries used) */
+  int32_t periodic[3];
+  int32_t stacked;          /* element order: 0 first dimension fastest, 1 last (vertical) dimension fastest */
+  const double* coords[3];  /* host arrays of Kd[d]+1 vertex coordinates */
+  const double* xi;         /* host: N+1 LGL nodes */
+  const double* w;          /* host: N+1 LGL weights */
+} atm_brick_desc;
+
+int atm_dg_create_brick(atm_ctx* ctx, const atm_dg_desc* d, const atm_brick_desc* b, atm_dg** out);
+
+/* Copies one geometry array of an operator to host memory, in the layout the
+ * operator description uses for it; POINTS (brick operators only) is dim
+ * blocks of Np*K node coordinates. */
+#define ATM_GEOM_POINTS 0
+#define ATM_GEOM_G 1
+#define ATM_GEOM_MJ 2
+#define ATM_GEOM_MJI 3
+#define ATM_GEOM_FACEMJ 4
+#define ATM_GEOM_NORMAL 5
+#define ATM_GEOM_FACEIX_P 6
+#define ATM_GEOM_BCTAG 7
+int atm_dg_geometry(atm_dg* dg, int which, void* host);
+
+/* ---- snapshot stream ---------------------------------------------------------
+ * The output hook of a time loop (solve!'s after_step callback,
+ * src/odesolvers.jl:9-26; VTK/state output in experiment/*) without stalling
+ * the compute stream.  atm_snapshot copies `bytes` from the device buffer src
+ * to pinned host memory host_dst (from atm_host_alloc): through the device
+ * staging buffer `staging` (same size) if given, so that the compute stream
+ * continues while a second stream of the context does the device-to-host
+ * copy; without staging the compute stream waits for the copy.  slot in
+ * [0, 16) names the copy: atm_snapshot_query returns 1 once it has landed (0
+ * before), atm_snapshot_wait blocks until then.  A slot's staging buffer may
+ * be reused by the next atm_snapshot on that slot (ordered on the device);
+ * host_dst must not be reused before the slot's copy has landed.  The query
+ * and wait calls may come from another host thread. */
+int atm_host_alloc(atm_ctx* ctx, size_t bytes, void** host);
+int atm_host_free(atm_ctx* ctx, void* host);
+int atm_snapshot(atm_ctx* ctx, int slot, void* host_dst, void* staging, const void* src, size_t bytes);
+int atm_snapshot_query(atm_ctx* ctx, int slot);
+int atm_snapshot_wait(atm_ctx* ctx, int slot);
+
 /* ---- measures (src/measures.jl:6-33) ---------------------------------------
  * Reductions over all nodes on the device, in double-double arithmetic with a
  * fixed order (reproducible bit for bit); the result is copied to the host and

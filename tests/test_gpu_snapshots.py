@@ -1,0 +1,60 @@
+"""Snapshot stream on the device: states written by SnapshotWriter during a run
+equal, bit for bit, the states of plain runs of the same number of steps."""
+import numpy as np
+import pytest
+
+import problems as P
+
+pytestmark = pytest.mark.gpu
+
+
+def _vortex():
+    import atum_hip as A
+    K, N = 8, 4
+    v = P.julia_range(-5, 5, K + 1)
+    grid = A.brickgrid(A.LobattoCell(N, 2), (v, v), (True, True))
+    dg = A.DGSEM(law=A.EulerLaw(2), grid=grid,
+                 volume_form=A.FluxDifferencingForm(A.EntropyConservativeFlux()),
+                 surface_numericalflux=A.RusanovFlux(), precision="fast")
+    q0 = P.isentropic_vortex(grid.points, 0.0, center=(0.0, 0.0), u0=(1.0, 1.0))
+    return dg, q0, 0.1 * (10 / K) / N
+
+
+@pytest.mark.parametrize("ring", [1, 3])
+def test_snapshots_equal_plain_runs(tmp_path, ring):
+    import atum_hip as A
+    dg, q0, dt = _vortex()
+    q = dg.fieldarray(q0)
+    solver = A.LSRK54(dg, q, dt)
+    w = A.SnapshotWriter(q, str(tmp_path / "v_{step:04d}.atm"), every=5, ring=ring,
+                         meta={"law": "euler"}, solver=solver)
+    for step in range(1, 21):  # solve!'s loop without the clipped final step
+        t = solver.time
+        solver.dostep(q)
+        w(step, t, q)
+    files = w.close()
+    assert len(files) == 4
+    for f in sorted(files):
+        got, head = A.read_snapshot(f)
+        n = head["step"]
+        ref = dg.fieldarray(q0)
+        A.LSRK54(dg, ref, dt).run(ref, n)
+        dg.ctx.sync()
+        np.testing.assert_array_equal(got, ref.numpy())
+        assert head["law"] == "euler" and head["time"] == pytest.approx(n * dt, rel=1e-12)
+    np.testing.assert_array_equal(A.read_snapshot(sorted(files)[-1])[0], q.numpy())
+
+
+def test_snapshot_without_staging_and_query():
+    """Direct copy (no staging buffer): the compute stream waits for the copy."""
+    import atum_hip as A
+    from atum_hip import _lib
+    from atum_hip.snapshots import PinnedBuffer
+    dg, q0, dt = _vortex()
+    q = dg.fieldarray(q0)
+    host = PinnedBuffer(dg.ctx, q.shape)
+    dg.ctx.check(_lib.lib.atm_snapshot(dg.ctx.h, 5, host.ptr, None, q.ptr, q.nbytes), "snapshot")
+    A.LSRK54(dg, q, dt).run(q, 3)  # may start only after the copy has read q
+    dg.ctx.check(_lib.lib.atm_snapshot_wait(dg.ctx.h, 5), "wait")
+    assert _lib.lib.atm_snapshot_query(dg.ctx.h, 5) == 1
+    np.testing.assert_array_equal(host.array, q0)
