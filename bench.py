@@ -255,6 +255,33 @@ def serve_workload(args):
             "wall_s": s["wall_s"], "ticks": s["ticks"], "max_active": s["max_active"]}
 
 
+def copy_workload(args):
+    """Calibration of the HBM byte counters (and a practical bandwidth reference): three
+    device-to-device copies of 1 GiB through the library (atm_copy), timed with HIP events."""
+    import ctypes
+    import atum_hip._lib as L
+
+    ctx = L.context()
+    n = 1 << 30
+    a, b = ctypes.c_void_p(), ctypes.c_void_p()
+    ctx.check(L.lib.atm_alloc(ctx.h, n, ctypes.byref(a)), "atm_alloc")
+    ctx.check(L.lib.atm_alloc(ctx.h, n, ctypes.byref(b)), "atm_alloc")
+    ctx.check(L.lib.atm_copy(ctx.h, b, a, n), "atm_copy")  # warm-up
+    ctx.record(0)
+    for _ in range(3):
+        ctx.check(L.lib.atm_copy(ctx.h, b, a, n), "atm_copy")
+    ctx.record(1)
+    ms = ctx.elapsed_ms(0, 1)
+    ctx.sync()
+    L.lib.atm_free(ctx.h, a)
+    L.lib.atm_free(ctx.h, b)
+    gbs = 3 * 2 * n / (ms / 1e3) / 1e9
+    return {"metric": "device-to-device copy bandwidth (read + write)", "value": gbs, "unit": "GB/s",
+            "n_gpus": 1, "steps": 3, "warmup": 1, "ms_per_step": ms / 3, "higher_is_better": True,
+            "scaling": "weak", "vs_baseline": None, "dtype": "u8", "data": "synthetic",
+            "config": {"workload": "copy of 1 GiB, three times (HBM counter calibration)", "bytes": n}}
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -270,7 +297,7 @@ def main():
                          "reference: the reference's summation order, no FMA")
     ap.add_argument("--no-reference-order", action="store_true",
                     help="skip the extra timing of the reference-order kernel")
-    ap.add_argument("--workload", choices=["bubble", "ensemble", "imex", "vortex", "serve"], default="bubble",
+    ap.add_argument("--workload", choices=["bubble", "ensemble", "imex", "vortex", "serve", "copy"], default="bubble",
                     help="bubble (default, the metric's configuration); the others are extra lines")
     ap.add_argument("--instances", type=int, default=256)
     ap.add_argument("--serve-seconds", type=float, default=10.0)
@@ -288,7 +315,7 @@ def main():
 
     if args.workload != "bubble":
         fn = {"ensemble": ensemble_workload, "imex": imex_workload, "vortex": vortex_workload,
-              "serve": serve_workload}[args.workload]
+              "serve": serve_workload, "copy": copy_workload}[args.workload]
         print(json.dumps(fn(args)))
         return
 

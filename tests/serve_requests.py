@@ -13,10 +13,10 @@ def _vortex(K, N):
         key = ("vortex", K, N)
         if key not in cache:
             v = P.julia_range(-5, 5, K + 1)
-            grid = A.brickgrid(A.LobattoCell(N, 2), (v, v), (True, True))
+            grid = A.brickgrid(A.LobattoCell(N, 2), (v, v), (True, True), lazy=True)
             dg = A.DGSEM(law=A.EulerLaw(2), grid=grid,
                          volume_form=A.FluxDifferencingForm(A.EntropyConservativeFlux()),
-                         surface_numericalflux=A.RusanovFlux(), precision="fast")
+                         surface_numericalflux=A.RusanovFlux(), precision="fast", geometry="device")
             cache[key] = (grid, dg)
         grid, dg = cache[key]
         q0 = P.isentropic_vortex(grid.points, 0.0, center=(0.0, 0.0), u0=(1.0, 1.0))
@@ -30,8 +30,9 @@ def _sw(K, N, amp, w, c):
         key = ("sw", K, N)
         if key not in cache:
             v = P.julia_range(0.0, 1.0, K + 1)
-            grid = A.brickgrid(A.LobattoCell(N, 2), (v, v), (True, True))
-            dg = A.DGSEM(law=A.ShallowWaterLaw(2, grav=9.81), grid=grid, surface_numericalflux=A.RusanovFlux())
+            grid = A.brickgrid(A.LobattoCell(N, 2), (v, v), (True, True), lazy=True)
+            dg = A.DGSEM(law=A.ShallowWaterLaw(2, grav=9.81), grid=grid, surface_numericalflux=A.RusanovFlux(),
+                         geometry="device")
             cache[key] = (grid, dg)
         grid, dg = cache[key]
         return dg, P.sw_bump(grid.points, amp, w, c), P.sw_ensemble_dt(K, N, amp)
@@ -45,13 +46,13 @@ def _imex(KH, KV, N, seed):
         if key not in cache:
             vx = P.julia_range(0.0, 20e3, KH + 1)
             vz = P.julia_range(0.0, 10e3, KV + 1)
-            grid = A.brickgrid(A.LobattoCell(N, 3), (vx, vx, vz), (True, True, False), "stacked")
+            grid = A.brickgrid(A.LobattoCell(N, 3), (vx, vx, vz), (True, True, False), "stacked", lazy=True)
             prob = A.Problem(boundary="reflect", referencestate=lambda x: P.isothermal_reference(x))
             law = A.EulerGravityLaw(3, grav=9.81, problem=prob)
             ex = A.DGSEM(law=law, grid=grid, volume_form=A.FluxDifferencingForm(A.EntropyConservativeFlux()),
-                         surface_numericalflux=A.RusanovFlux(), precision="fast")
+                         surface_numericalflux=A.RusanovFlux(), precision="fast", geometry="device")
             lin = A.DGSEM(law=A.LinearEulerGravityLaw(law), grid=grid, surface_numericalflux=A.RusanovFlux(),
-                          directions=(3,), auxstate=ex.auxstate)
+                          directions=(3,), auxstate=ex.auxstate, geometry="device")
             cache[key] = (grid, ex, lin)
         grid, ex, lin = cache[key]
         q0 = P.theta_noise_state(grid, ex.auxstate, seed)
