@@ -59,13 +59,20 @@ class ServeStats:
     wall: float = 0.0
     ticks: int = 0
     max_active: int = 0
+    admit_s: float = 0.0     # host: building / uploading admitted requests
+    enqueue_s: float = 0.0   # host: launching a tick's work
+    wait_s: float = 0.0      # host: waiting for the device at the end of a tick
+    retire_s: float = 0.0    # host: downloading results, freeing
+    gpu_span_s: float = 0.0  # device: from the first to the last kernel of each tick (events)
 
     def summary(self):
         lat = np.array(self.latencies) if self.latencies else np.zeros(1)
         return {"completed": self.completed, "p50_s": float(np.percentile(lat, 50)),
                 "p99_s": float(np.percentile(lat, 99)), "mean_s": float(lat.mean()),
                 "dof_updates_per_s": self.dof_updates / self.wall if self.wall > 0 else 0.0,
-                "wall_s": self.wall, "ticks": self.ticks, "max_active": self.max_active}
+                "wall_s": self.wall, "ticks": self.ticks, "max_active": self.max_active,
+                "admit_s": self.admit_s, "enqueue_s": self.enqueue_s, "wait_s": self.wait_s,
+                "retire_s": self.retire_s, "gpu_span_s": self.gpu_span_s}
 
 
 class Server:
@@ -122,11 +129,13 @@ class Server:
         pi = 0
         while pi < len(pending) or active:
             now = time.perf_counter() - t0
+            ta = time.perf_counter()
             while (pi < len(pending) and pending[pi].arrival <= now and len(active) < self.max_active
                    and self.in_use + pending[pi].footprint <= self.hbm_budget):
                 self._admit(pending[pi], time.perf_counter() - t0)
                 active.append(pending[pi])
                 pi += 1
+            stats.admit_s += time.perf_counter() - ta
             if not active:
                 if pi < len(pending):
                     time.sleep(max(0.0, min(0.01, pending[pi].arrival - now)))
@@ -135,6 +144,8 @@ class Server:
             explicit = [r for r in active if r.kind != "imex"]
             implicit = [r for r in active if r.kind == "imex"]
             step = min(self.chunk, min(r._left for r in active))
+            te = time.perf_counter()
+            self.ctx.record(14)
             if explicit:
                 ids = [r.rid for r in explicit]
                 if ids != members:
@@ -144,15 +155,22 @@ class Server:
                                [r._scratch for r in explicit], [r._dt for r in explicit], step, LSRK54)
             for r in implicit:
                 r._ark.run(r._q, step)
+            self.ctx.record(15)
+            tw = time.perf_counter()
+            stats.enqueue_s += tw - te
             self.ctx.sync()
+            stats.wait_s += time.perf_counter() - tw
+            stats.gpu_span_s += self.ctx.elapsed_ms(14, 15) / 1e3
             stats.ticks += 1
             for r in active:
                 r._left -= step
                 stats.dof_updates += r.dof * step * (3 if r.kind == "imex" else 5)
             now = time.perf_counter() - t0
             done = [r for r in active if r._left == 0]
+            tr = time.perf_counter()
             for r in done:
                 self._retire(r, now, stats)
+            stats.retire_s += time.perf_counter() - tr
             active = [r for r in active if r._left > 0]
             if on_tick:
                 on_tick(stats, active)

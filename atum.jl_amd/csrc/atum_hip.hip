@@ -6,6 +6,7 @@
 #include <cstdio>
 #include <cstring>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "handles.hpp"
@@ -24,6 +25,7 @@ struct atm_batch {
   };
   std::vector<atm_dg*> dgs;
   std::vector<Group> groups;
+  hipGraphExec_t exec[2] = {nullptr, nullptr};  // two-step stage sequence per scheme (reads the tables)
 };
 
 struct atm_banded {
@@ -37,6 +39,13 @@ struct atm_banded {
   int32_t* perm = nullptr;      // lines ordered by matrix, each matrix's run padded to 64 with -1
   int32_t* wave_mat = nullptr;  // matrix of each run of 64
   int64_t nwaves = 0;
+  // block form of the factors (element blocks of BS = bw+1 rows): per matrix u and block i,
+  // [L_{i,i-1} | L_ii^-1 | U_{i,i+1} | U_ii^-1], each BS x BS row-major (null: not built)
+  double* blocks = nullptr;
+  double* frags = nullptr;        // the block form as MFMA A-operand fragments (band_frags_kernel)
+  int32_t* perm16 = nullptr;      // as perm, runs padded to 16 lines
+  int32_t* wave_mat16 = nullptr;
+  int64_t nwaves16 = 0;
   int32_t* elem_col = nullptr;  // column of each element (-1: none)
   int32_t* elem_ez = nullptr;   // position of each element in its column
   int32_t* colelem = nullptr;   // ncols*KV
@@ -67,27 +76,33 @@ __global__ void copy_kernel(double* __restrict__ y, const double* __restrict__ x
     y[i] = x[i];
 }
 
-// y -= x
-__global__ void sub_kernel(double* __restrict__ y, const double* __restrict__ x, int64_t n) {
-  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
-    y[i] = y[i] - x[i];
-}
 
 struct ArkCoef {
   double ea[3], ia[3];
 };
 
-// Qhat = q; Qhat += dt*(a_k*exK_k + ã_k*imK_k) for k < nk (src/odesolvers.jl:346-349)
+// Qhat = q; Qhat += dt*(a_k*exK_k + ã_k*imK_k) for k < nk (src/odesolvers.jl:346-349).
+// unsplit: exK_k holds the full tendency and the explicit part is exK_k - imK_k
+// (src/odesolvers.jl:339-341, 374-376), formed here with the same rounding.
 __global__ void ark_combine_kernel(double* out, const double* q, double dt,
                                    int nk, ArkCoef c, const double* __restrict__ e0,
                                    const double* __restrict__ e1, const double* __restrict__ e2,
                                    const double* __restrict__ i0, const double* __restrict__ i1,
-                                   const double* __restrict__ i2, int64_t n) {
+                                   const double* __restrict__ i2, int64_t n, int unsplit) {
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
     double v = q[i];
-    if (nk > 0) v = v + dt * (c.ea[0] * e0[i] + c.ia[0] * i0[i]);
-    if (nk > 1) v = v + dt * (c.ea[1] * e1[i] + c.ia[1] * i1[i]);
-    if (nk > 2) v = v + dt * (c.ea[2] * e2[i] + c.ia[2] * i2[i]);
+    if (nk > 0) {
+      const double im = i0[i], ex = unsplit ? e0[i] - im : e0[i];
+      v = v + dt * (c.ea[0] * ex + c.ia[0] * im);
+    }
+    if (nk > 1) {
+      const double im = i1[i], ex = unsplit ? e1[i] - im : e1[i];
+      v = v + dt * (c.ea[1] * ex + c.ia[1] * im);
+    }
+    if (nk > 2) {
+      const double im = i2[i], ex = unsplit ? e2[i] - im : e2[i];
+      v = v + dt * (c.ea[2] * ex + c.ia[2] * im);
+    }
     out[i] = v;
   }
 }
@@ -505,6 +520,376 @@ band_solve_reg_kernel(BandGeo g, const double* __restrict__ fac, const int32_t* 
   }
 }
 
+// Block form of one factored matrix u, element block i (one workgroup per (u, i)):
+// the coupling blocks L_{i,i-1} (entries with p > m) and U_{i,i+1} (p < m), and the
+// inverses of the unit lower L_ii and upper U_ii by substitution on identity
+// columns (one thread per column).
+__global__ void band_blocks_kernel(BandGeo g, const double* __restrict__ fac, double* __restrict__ blocks) {
+  const int64_t ui = blockIdx.x;
+  const int64_t u = ui / g.KV;
+  const int i = (int)(ui - u * g.KV);
+  const int BS = g.bw + 1, nb = g.nb, bw = g.bw, BB = BS * BS;
+  const double* F = fac + u * ((int64_t)g.n * nb);
+  double* B = blocks + ui * 4 * (int64_t)BB;
+  auto A = [&](int r, int c) -> double { return F[(int64_t)r * nb + bw + c - r]; };
+  for (int t = threadIdx.x; t < BB; t += blockDim.x) {
+    const int m = t / BS, p = t - m * BS, r = i * BS + m;
+    B[t] = (i > 0 && p > m) ? A(r, (i - 1) * BS + p) : 0.0;
+    B[2 * BB + t] = (i + 1 < g.KV && p < m) ? A(r, (i + 1) * BS + p) : 0.0;
+  }
+  const int c = threadIdx.x;
+  if (c < BS) {
+    double* Li = B + BB;
+    double* Ui = B + 3 * BB;
+    for (int m = 0; m < BS; ++m) {
+      double v = (m == c) ? 1.0 : 0.0;
+      for (int k = c; k < m; ++k) v -= A(i * BS + m, i * BS + k) * Li[k * BS + c];
+      Li[m * BS + c] = v;
+    }
+    for (int m = BS - 1; m >= 0; --m) {
+      double v = (m == c) ? 1.0 : 0.0;
+      for (int k = m + 1; k <= c; ++k) v -= A(i * BS + m, i * BS + k) * Ui[k * BS + c];
+      Ui[m * BS + c] = v / A(i * BS + m, i * BS + m);
+    }
+  }
+}
+
+// Banded solve in block form: per element block of rows, forward
+//   y_i = L_ii^-1 (b_i - L_{i,i-1} y_{i-1}),   backward  x_i = U_ii^-1 (y_i - U_{i,i+1} x_{i+1}),
+// two dense BS x BS products per block with no serial chain.  A wave takes 16
+// lines (grouped by matrix, so one matrix per wave), 4 lanes per line, each
+// lane forming ceil(BS/4) rows of every product; the vector of a line is
+// exchanged through LDS between the products.  The next block's matrices are
+// loaded while the current block is computed and staged in LDS (read by
+// broadcast).  Agrees with substitution to round-off.
+template <int NS, int NQ>
+__global__ void __launch_bounds__(64)
+band_solve_blk_kernel(BandGeo g, const double* __restrict__ blocks, const int32_t* __restrict__ perm,
+                      const int32_t* __restrict__ wave_mat, double* x, const double* b) {
+#pragma clang fp contract(fast)
+  constexpr int BS = NS * NQ, PARTS = 4, LPW = 64 / PARTS, RPL = (BS + PARTS - 1) / PARTS;
+  constexpr int BB = BS * BS, FB = 2 * BB, FPL = (FB + 63) / 64;
+  __shared__ double mats[2][FB];
+  __shared__ double vec[LPW][BS + 1];
+  const int lane = threadIdx.x, j = lane / PARTS, part = lane % PARTS;
+  const int32_t pl = perm[blockIdx.x * (int64_t)LPW + j];
+  const bool active = pl >= 0;
+  const int64_t line = active ? pl : 0;  // padding lanes read line 0 and store nothing
+  const int KV = g.KV;
+  const double* Bu = blocks + (int64_t)wave_mat[blockIdx.x] * KV * 2 * FB;  // block i: [fwd FB | bwd FB]
+  const int64_t c = line / g.Nh;
+  const int h = (int)(line - c * g.Nh);
+  const int32_t* col = g.colelem + c * KV;
+  auto gi = [&](int ez, int m) -> int64_t {
+    return (int64_t)(m % NS) * g.Ntot + h + (int64_t)g.Np * col[ez] + (int64_t)g.Nh * (m / NS);
+  };
+  double pf[FPL];
+  auto fetch = [&](int ez, int half) {
+#pragma unroll
+    for (int i = 0; i < FPL; ++i) {
+      const int k = lane + 64 * i;
+      if (k < FB) pf[i] = Bu[((int64_t)ez * 2 + half) * FB + k];
+    }
+  };
+  auto stage = [&](int buf) {
+#pragma unroll
+    for (int i = 0; i < FPL; ++i) {
+      const int k = lane + 64 * i;
+      if (k < FB) mats[buf][k] = pf[i];
+    }
+  };
+  auto load_rows = [&](const double* src, int ez, double* dst) {
+#pragma unroll
+    for (int k = 0; k < RPL; ++k) {
+      const int m = part * RPL + k;
+      dst[k] = (m < BS) ? src[gi(ez, m)] : 0.0;
+    }
+  };
+  double v[BS], rb[RPL], nrb[RPL], t[RPL];
+  for (int pass = 0; pass < 2; ++pass) {
+    const double* src = pass == 0 ? b : x;
+    const int first = pass == 0 ? 0 : KV - 1, step = pass == 0 ? 1 : -1;
+    fetch(first, pass);
+    stage(0);
+    load_rows(src, first, nrb);
+    __syncthreads();
+    for (int it = 0, ez = first; it < KV; ++it, ez += step) {
+      const int buf = it & 1;
+#pragma unroll
+      for (int k = 0; k < RPL; ++k) rb[k] = nrb[k];
+      if (it + 1 < KV) {
+        fetch(ez + step, pass);
+        load_rows(src, ez + step, nrb);
+      }
+      const double* Cm = mats[buf];       // coupling block
+      const double* Iv = mats[buf] + BB;  // inverse of the diagonal block
+      // t = r - C v (v: the neighbouring block's solution); rows past BS (when 4 does
+      // not divide BS) compute on row BS-1 and are never stored
+#pragma unroll
+      for (int k = 0; k < RPL; ++k) {
+        const int m = part * RPL + k < BS ? part * RPL + k : BS - 1;
+        double s = rb[k];
+        if (it > 0) {
+#pragma unroll
+          for (int p = 0; p < BS; ++p) s -= Cm[m * BS + p] * v[p];
+        }
+        t[k] = s;
+      }
+#pragma unroll
+      for (int k = 0; k < RPL; ++k) {
+        const int m = part * RPL + k;
+        if (m < BS) vec[j][m] = t[k];
+      }
+      __syncthreads();
+#pragma unroll
+      for (int p = 0; p < BS; ++p) v[p] = vec[j][p];
+      // y = D^-1 t
+#pragma unroll
+      for (int k = 0; k < RPL; ++k) {
+        const int m = part * RPL + k < BS ? part * RPL + k : BS - 1;
+        double s = 0.0;
+#pragma unroll
+        for (int p = 0; p < BS; ++p) s += Iv[m * BS + p] * v[p];
+        t[k] = s;
+      }
+      __syncthreads();
+#pragma unroll
+      for (int k = 0; k < RPL; ++k) {
+        const int m = part * RPL + k;
+        if (m < BS) {
+          vec[j][m] = t[k];
+          if (active) x[gi(ez, m)] = t[k];
+        }
+      }
+      if (it + 1 < KV) stage(1 - buf);
+      __syncthreads();
+#pragma unroll
+      for (int p = 0; p < BS; ++p) v[p] = vec[j][p];
+    }
+    __syncthreads();
+  }
+}
+
+// Fragments of the block form for v_mfma_f64_16x16x4f64, per matrix u, element
+// block i and matrix mi (0: -L_{i,i-1}, 1: L_ii^-1, 2: -U_{i,i+1}, 3: U_ii^-1):
+// fragment (ti, ks) holds, for lane l, M[16 ti + (l & 15)][4 ks + (l >> 4)] (zero
+// outside BS x BS), i.e. the A operand of one MFMA, 64 doubles in lane order.
+__global__ void band_frags_kernel(BandGeo g, const double* __restrict__ blocks, double* __restrict__ frags, int TI,
+                                  int KS) {
+  const int64_t ui = blockIdx.x;
+  const int BS = g.bw + 1, BB = BS * BS, NF = TI * KS;
+  const double* B = blocks + ui * 4 * (int64_t)BB;
+  double* F = frags + ui * 4 * (int64_t)NF * 64;
+  for (int t = threadIdx.x; t < 4 * NF * 64; t += blockDim.x) {
+    const int l = t % 64, f = (t / 64) % NF, mi = t / (64 * NF);
+    const int row = 16 * (f / KS) + (l & 15), colk = 4 * (f % KS) + (l >> 4);
+    double v = (row < BS && colk < BS) ? B[mi * BB + row * BS + colk] : 0.0;
+    F[t] = (mi == 0 || mi == 2) ? -v : v;
+  }
+}
+
+typedef double atm_v4d __attribute__((ext_vector_type(4)));
+
+// Banded solve in block form on the matrix cores: a wave takes 16 lines (one
+// matrix per wave), line j = lane & 15.  Per block of rows the line vector is
+// held in B-operand order (lane holds rows 4 ks + (lane >> 4)), which is also
+// the order of the MFMA result rows (16 ti + (lane >> 4) + 4 r), so the two
+// products
+//   t = r - C v  (C negated in the fragments, r as accumulator),  v = D^-1 t
+// run back to back in registers.  Forward pass: right-hand-side rows and the
+// (L2-resident) operator fragments are loaded two blocks ahead into fixed
+// register buffers; the forward solution is kept in LDS (yb[r * 17 + j]).
+// Backward pass: rows come from LDS, each finished block is stored at once.
+template <int NS, int NQ>
+__global__ void __launch_bounds__(64)
+band_solve_mfma_kernel(BandGeo g, const double* __restrict__ frags, const int32_t* __restrict__ perm,
+                       const int32_t* __restrict__ wave_mat, double* x, const double* b) {
+  constexpr int BS = NS * NQ, TI = (BS + 15) / 16, KS = (BS + 3) / 4, NF = TI * KS, LD = 17;
+  constexpr bool FULL = BS % 4 == 0;  // every lane's rows 4 ks + qq lie inside the block
+  extern __shared__ double yb[];
+  const int l = threadIdx.x, j = l & 15, qq = l >> 4;
+  const int32_t pl = perm[blockIdx.x * (int64_t)16 + j];
+  const bool active = pl >= 0;
+  const int64_t line = active ? pl : 0;  // padding lanes read line 0 and store nothing
+  const int KV = g.KV, n = g.n;
+  const double* Fu = frags + (int64_t)wave_mat[blockIdx.x] * KV * 4 * NF * 64;
+  const int64_t c = line / g.Nh;
+  const int h = (int)(line - c * g.Nh);
+  // the elements of each line's column, staged in LDS: cl[j * KV + ez]
+  int32_t* cl = reinterpret_cast<int32_t*>(yb + (size_t)n * LD);
+  for (int ez = qq; ez < KV; ez += 4) cl[j * KV + ez] = g.colelem[c * KV + ez];
+  __syncthreads();
+  auto gaddr = [&](int ez, int m) -> int64_t {
+    return (int64_t)(m % NS) * g.Ntot + h + (int64_t)g.Nh * (m / NS) + (int64_t)g.Np * cl[j * KV + ez];
+  };
+  auto row_ok = [&](int m) -> bool { return FULL || m < BS; };
+  double v[KS];
+  double f0[2 * NF], f1[2 * NF], f2[2 * NF];  // fragments of three consecutive blocks (fixed buffers)
+  double r0[KS], r1[KS], r2[KS];              // right-hand-side rows of the same blocks (forward pass)
+  auto fetch = [&](int ez, int m0, double* dst) {
+#pragma unroll
+    for (int f = 0; f < 2 * NF; ++f) dst[f] = Fu[((int64_t)(ez * 4 + m0) * NF + f) * 64 + l];
+  };
+  auto fetch_rows = [&](int ez, double* dst) {
+#pragma unroll
+    for (int ks = 0; ks < KS; ++ks) {
+      const int m = 4 * ks + qq;
+      dst[ks] = row_ok(m) ? b[gaddr(ez, m)] : 0.0;
+    }
+  };
+  auto lds_rows = [&](int ez, double* dst) {
+#pragma unroll
+    for (int ks = 0; ks < KS; ++ks) {
+      const int m = 4 * ks + qq;
+      dst[ks] = row_ok(m) ? yb[(ez * BS + m) * LD + j] : 0.0;
+    }
+  };
+  // one block of rows: t = r - C v (it > 0), v = D^-1 t.  Each tile's k-sum is
+  // split over two accumulators (even and odd k steps) for more independent
+  // MFMA chains.
+  auto block = [&](int it, int ez, const double* fa, const double* rr, int pass) {
+    double t[KS];
+    if (it > 0) {
+      atm_v4d a0[TI], a1[TI];
+#pragma unroll
+      for (int ti = 0; ti < TI; ++ti)
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          a0[ti][q] = (4 * ti + q < KS) ? rr[(4 * ti + q) < KS ? 4 * ti + q : 0] : 0.0;
+          a1[ti][q] = 0.0;
+        }
+#pragma unroll
+      for (int ks = 0; ks < KS; ++ks)
+#pragma unroll
+        for (int ti = 0; ti < TI; ++ti) {
+          if (ks % 2 == 0) a0[ti] = __builtin_amdgcn_mfma_f64_16x16x4f64(fa[ti * KS + ks], v[ks], a0[ti], 0, 0, 0);
+          else a1[ti] = __builtin_amdgcn_mfma_f64_16x16x4f64(fa[ti * KS + ks], v[ks], a1[ti], 0, 0, 0);
+        }
+#pragma unroll
+      for (int ti = 0; ti < TI; ++ti)
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+          if (4 * ti + q < KS) t[4 * ti + q] = a0[ti][q] + a1[ti][q];
+    } else {
+#pragma unroll
+      for (int ks = 0; ks < KS; ++ks) t[ks] = rr[ks];
+    }
+    {
+      atm_v4d a0[TI], a1[TI];
+#pragma unroll
+      for (int ti = 0; ti < TI; ++ti)
+#pragma unroll
+        for (int q = 0; q < 4; ++q) a0[ti][q] = a1[ti][q] = 0.0;
+#pragma unroll
+      for (int ks = 0; ks < KS; ++ks)
+#pragma unroll
+        for (int ti = 0; ti < TI; ++ti) {
+          if (ks % 2 == 0) a0[ti] = __builtin_amdgcn_mfma_f64_16x16x4f64(fa[NF + ti * KS + ks], t[ks], a0[ti], 0, 0, 0);
+          else a1[ti] = __builtin_amdgcn_mfma_f64_16x16x4f64(fa[NF + ti * KS + ks], t[ks], a1[ti], 0, 0, 0);
+        }
+#pragma unroll
+      for (int ti = 0; ti < TI; ++ti)
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+          if (4 * ti + q < KS) v[4 * ti + q] = a0[ti][q] + a1[ti][q];
+    }
+#pragma unroll
+    for (int ks = 0; ks < KS; ++ks) {
+      const int m = 4 * ks + qq;
+      if (row_ok(m)) {
+        if (pass == 0) yb[(ez * BS + m) * LD + j] = v[ks];
+        else if (active) x[gaddr(ez, m)] = v[ks];
+      }
+    }
+  };
+#pragma unroll
+  for (int ks = 0; ks < KS; ++ks) v[ks] = 0.0;
+  // forward: L y = b, blocks it, it+1, it+2 in buffers 0, 1, 2, each refilled two blocks ahead
+  fetch(0, 0, f0);
+  fetch_rows(0, r0);
+  if (KV > 1) {
+    fetch(1, 0, f1);
+    fetch_rows(1, r1);
+  }
+  for (int it = 0; it < KV; it += 3) {
+    if (it + 2 < KV) {
+      fetch(it + 2, 0, f2);
+      fetch_rows(it + 2, r2);
+    }
+    block(it, it, f0, r0, 0);
+    if (it + 1 >= KV) break;
+    if (it + 3 < KV) {
+      fetch(it + 3, 0, f0);
+      fetch_rows(it + 3, r0);
+    }
+    block(it + 1, it + 1, f1, r1, 0);
+    if (it + 2 >= KV) break;
+    if (it + 4 < KV) {
+      fetch(it + 4, 0, f1);
+      fetch_rows(it + 4, r1);
+    }
+    block(it + 2, it + 2, f2, r2, 0);
+  }
+  __syncthreads();
+  // backward: U x = y, rows from LDS, solution stored per block
+  fetch(KV - 1, 2, f0);
+  if (KV > 1) fetch(KV - 2, 2, f1);
+  for (int it = 0; it < KV; it += 3) {
+    const int ez = KV - 1 - it;
+    if (it + 2 < KV) fetch(ez - 2, 2, f2);
+    lds_rows(ez, r0);
+    block(it, ez, f0, r0, 1);
+    if (it + 1 >= KV) break;
+    if (it + 3 < KV) fetch(ez - 3, 2, f0);
+    lds_rows(ez - 1, r1);
+    block(it + 1, ez - 1, f1, r1, 1);
+    if (it + 2 >= KV) break;
+    if (it + 4 < KV) fetch(ez - 4, 2, f1);
+    lds_rows(ez - 2, r2);
+    block(it + 2, ez - 2, f2, r2, 1);
+  }
+}
+
+size_t band_mfma_lds_bytes(int n, int KV) { return (size_t)n * 17 * sizeof(double) + (size_t)16 * KV * sizeof(int32_t); }
+
+typedef void (*BandSolveMfmaFn)(BandGeo, const double*, const int32_t*, const int32_t*, double*, const double*);
+
+BandSolveMfmaFn band_mfma_kernel(int Ns, int Nq) {
+#define ATM_BAND_NQ(NS)                                \
+  switch (Nq) {                                        \
+    case 2: return band_solve_mfma_kernel<NS, 2>;      \
+    case 3: return band_solve_mfma_kernel<NS, 3>;      \
+    case 4: return band_solve_mfma_kernel<NS, 4>;      \
+    case 5: return band_solve_mfma_kernel<NS, 5>;      \
+    case 6: return band_solve_mfma_kernel<NS, 6>;      \
+    default: return nullptr;                           \
+  }
+  if (Ns == 1) ATM_BAND_NQ(1)
+  if (Ns == 4) ATM_BAND_NQ(4)
+  if (Ns == 5) ATM_BAND_NQ(5)
+#undef ATM_BAND_NQ
+  return nullptr;
+}
+
+typedef void (*BandSolveBlkFn)(BandGeo, const double*, const int32_t*, const int32_t*, double*, const double*);
+
+BandSolveBlkFn band_blk_kernel(int Ns, int Nq) {
+#define ATM_BAND_NQ(NS)                                \
+  switch (Nq) {                                        \
+    case 2: return band_solve_blk_kernel<NS, 2>;       \
+    case 3: return band_solve_blk_kernel<NS, 3>;       \
+    case 4: return band_solve_blk_kernel<NS, 4>;       \
+    case 5: return band_solve_blk_kernel<NS, 5>;       \
+    case 6: return band_solve_blk_kernel<NS, 6>;       \
+    default: return nullptr;                           \
+  }
+  if (Ns == 1) ATM_BAND_NQ(1)
+  if (Ns == 4) ATM_BAND_NQ(4)
+  if (Ns == 5) ATM_BAND_NQ(5)
+#undef ATM_BAND_NQ
+  return nullptr;
+}
+
 typedef void (*BandSolveRegFn)(BandGeo, const double*, const int32_t*, const int32_t*, double*, const double*);
 
 // register-blocked kernels for the state counts of the vertical linear laws
@@ -902,6 +1287,7 @@ int atm_dg_destroy(atm_dg* dg) {
   if (!dg) return ATM_ERR_ARG;
   hipSetDevice(dg->ctx->device);
   hipStreamSynchronize(dg->ctx->stream);
+  if (dg->lsrk_exec) hipGraphExecDestroy(dg->lsrk_exec);
   for (void* p : dg->allocs) hipFree(p);
   delete dg;
   return ATM_OK;
@@ -949,31 +1335,39 @@ int atm_lsrk_run(atm_dg* dg, int scheme, double* q, double* dq, double* scratch,
   const int64_t pair = (t.ns % 2 == 1) ? 2 : 1;
   int64_t done = 0;
   if (use_graph && nsteps >= pair) {
+    // reuse the last instantiated graph when the buffers, dt and scheme are the same
     hipStream_t s = dg->ctx->stream;
-    hipGraph_t graph = nullptr;
-    hipGraphExec_t exec = nullptr;
-    HIPCHK(dg->ctx, hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-    int rc = ATM_OK;
-    for (int64_t i = 0; i < pair && rc == ATM_OK; ++i) rc = one_step();
-    hipError_t ec = hipStreamEndCapture(s, &graph);
-    if (rc) {
-      if (graph) hipGraphDestroy(graph);
-      return rc;
+    if (dg->lsrk_exec && (dg->lsrk_scheme != scheme || dg->lsrk_q != q || dg->lsrk_dq != dq ||
+                          dg->lsrk_scratch != scratch || dg->lsrk_dt != dt)) {
+      HIPCHK(dg->ctx, hipStreamSynchronize(s));
+      hipGraphExecDestroy(dg->lsrk_exec);
+      dg->lsrk_exec = nullptr;
     }
-    if (ec != hipSuccess) return hipfail(dg->ctx, ec, "hipStreamEndCapture");
-    HIPCHK(dg->ctx, hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
-    const int64_t reps = nsteps / pair;
-    for (int64_t i = 0; i < reps; ++i) {
-      hipError_t e = hipGraphLaunch(exec, s);
-      if (e != hipSuccess) {
-        hipGraphExecDestroy(exec);
-        hipGraphDestroy(graph);
-        return hipfail(dg->ctx, e, "hipGraphLaunch");
+    if (!dg->lsrk_exec) {
+      hipGraph_t graph = nullptr;
+      HIPCHK(dg->ctx, hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+      int rc = ATM_OK;
+      for (int64_t i = 0; i < pair && rc == ATM_OK; ++i) rc = one_step();
+      hipError_t ec = hipStreamEndCapture(s, &graph);
+      if (rc) {
+        if (graph) hipGraphDestroy(graph);
+        return rc;
       }
+      if (ec != hipSuccess) return hipfail(dg->ctx, ec, "hipStreamEndCapture");
+      hipGraphExec_t exec = nullptr;
+      hipError_t ei = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+      hipGraphDestroy(graph);
+      if (ei != hipSuccess) return hipfail(dg->ctx, ei, "hipGraphInstantiate");
+      dg->lsrk_exec = exec;
+      dg->lsrk_scheme = scheme;
+      dg->lsrk_q = q;
+      dg->lsrk_dq = dq;
+      dg->lsrk_scratch = scratch;
+      dg->lsrk_dt = dt;
     }
+    const int64_t reps = nsteps / pair;
+    for (int64_t i = 0; i < reps; ++i) HIPCHK(dg->ctx, hipGraphLaunch(dg->lsrk_exec, s));
     done = reps * pair;
-    hipGraphExecDestroy(exec);
-    hipGraphDestroy(graph);
   }
   for (; done < nsteps; ++done) {
     int rc = one_step();
@@ -1154,6 +1548,10 @@ int atm_batch_create(atm_ctx* ctx, int32_t n, atm_dg* const* members, atm_batch*
 
 int atm_batch_destroy(atm_batch* b) {
   if (!b) return ATM_ERR_ARG;
+  hipSetDevice(b->ctx->device);
+  hipStreamSynchronize(b->ctx->stream);
+  for (hipGraphExec_t& x : b->exec)
+    if (x) hipGraphExecDestroy(x);
   for (auto& G : b->groups)
     if (G.dtab) hipFree(G.dtab);
   delete b;
@@ -1208,31 +1606,31 @@ int atm_batch_lsrk_run(atm_batch* b, int scheme, double* const* q, double* const
   const int64_t pair = (t.ns % 2 == 1) ? 2 : 1;
   int64_t done = 0;
   if (use_graph && nsteps >= pair) {
+    // the captured kernels read the member tables, so one instantiated graph per
+    // scheme serves every later call on this batch
     hipStream_t s = ctx->stream;
-    hipGraph_t graph = nullptr;
-    hipGraphExec_t exec = nullptr;
-    HIPCHK(ctx, hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-    int rc = ATM_OK;
-    for (int64_t i = 0; i < pair && rc == ATM_OK; ++i) rc = one_step();
-    hipError_t ec = hipStreamEndCapture(s, &graph);
-    if (rc) {
-      if (graph) hipGraphDestroy(graph);
-      return rc;
-    }
-    if (ec != hipSuccess) return hipfail(ctx, ec, "hipStreamEndCapture");
-    HIPCHK(ctx, hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
-    const int64_t reps = nsteps / pair;
-    for (int64_t i = 0; i < reps; ++i) {
-      hipError_t e = hipGraphLaunch(exec, s);
-      if (e != hipSuccess) {
-        hipGraphExecDestroy(exec);
-        hipGraphDestroy(graph);
-        return hipfail(ctx, e, "hipGraphLaunch");
+    hipGraphExec_t& exec = b->exec[scheme == ATM_LSRK54 ? 0 : 1];
+    if (!exec) {
+      hipGraph_t graph = nullptr;
+      HIPCHK(ctx, hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+      int rc = ATM_OK;
+      for (int64_t i = 0; i < pair && rc == ATM_OK; ++i) rc = one_step();
+      hipError_t ec = hipStreamEndCapture(s, &graph);
+      if (rc) {
+        if (graph) hipGraphDestroy(graph);
+        return rc;
+      }
+      if (ec != hipSuccess) return hipfail(ctx, ec, "hipStreamEndCapture");
+      hipError_t ei = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+      hipGraphDestroy(graph);
+      if (ei != hipSuccess) {
+        exec = nullptr;
+        return hipfail(ctx, ei, "hipGraphInstantiate");
       }
     }
+    const int64_t reps = nsteps / pair;
+    for (int64_t i = 0; i < reps; ++i) HIPCHK(ctx, hipGraphLaunch(exec, s));
     done = reps * pair;
-    hipGraphExecDestroy(exec);
-    hipGraphDestroy(graph);
   }
   for (; done < nsteps; ++done) {
     int rc = one_step();
@@ -1403,6 +1801,25 @@ int atm_banded_create(atm_dg* lin, int32_t KV, const int32_t* colelem, int32_t n
     }
   }
   b->nwaves = (int64_t)wmat.size();
+  std::vector<int32_t> perm16, wmat16;
+  {
+    std::vector<std::vector<int32_t>> by(b->nu);
+    for (int64_t l = 0; l < L; ++l) by[b->line_mat_host[l]].push_back((int32_t)l);
+    for (int64_t u = 0; u < b->nu; ++u)
+      for (size_t i = 0; i < by[u].size(); i += 16) {
+        for (size_t k = i; k < i + 16; ++k) perm16.push_back(k < by[u].size() ? by[u][k] : -1);
+        wmat16.push_back((int32_t)u);
+      }
+  }
+  b->nwaves16 = (int64_t)wmat16.size();
+  if ((e = hipMalloc(&b->perm16, perm16.size() * sizeof(int32_t) + 8)) != hipSuccess ||
+      (e = hipMalloc(&b->wave_mat16, wmat16.size() * sizeof(int32_t) + 8)) != hipSuccess)
+    return done(hipfail(ctx, e, "hipMalloc"));
+  if (!perm16.empty() &&
+      ((e = hipMemcpy(b->perm16, perm16.data(), perm16.size() * sizeof(int32_t), hipMemcpyHostToDevice)) != hipSuccess ||
+       (e = hipMemcpy(b->wave_mat16, wmat16.data(), wmat16.size() * sizeof(int32_t), hipMemcpyHostToDevice)) !=
+           hipSuccess))
+    return done(hipfail(ctx, e, "hipMemcpy"));
   if ((e = hipMalloc(&b->perm, perm.size() * sizeof(int32_t) + 8)) != hipSuccess ||
       (e = hipMalloc(&b->wave_mat, wmat.size() * sizeof(int32_t) + 8)) != hipSuccess)
     return done(hipfail(ctx, e, "hipMalloc"));
@@ -1426,6 +1843,20 @@ int atm_banded_create(atm_dg* lin, int32_t KV, const int32_t* colelem, int32_t n
     else
       hipLaunchKernelGGL(band_lu_wg_kernel<false>, dim3(b->nu), dim3(LU_THREADS), 0, s, g, b->fac, b->nu);
     if ((e = hipGetLastError()) != hipSuccess) return done(hipfail(ctx, e, "band LU launch"));
+    // block form for the block solve (up to 2 GiB; beyond that the register-blocked substitution)
+    const size_t blk_bytes = (size_t)b->nu * b->KV * 4 * (size_t)(b->bw + 1) * (b->bw + 1) * sizeof(double);
+    if (band_blk_kernel(lin->Ns, lin->Nq) && blk_bytes <= ((size_t)1 << 31)) {
+      if ((e = hipMalloc(&b->blocks, blk_bytes)) != hipSuccess) return done(hipfail(ctx, e, "hipMalloc blocks"));
+      hipLaunchKernelGGL(band_blocks_kernel, dim3(b->nu * b->KV), dim3(64), 0, s, g, b->fac, b->blocks);
+      if ((e = hipGetLastError()) != hipSuccess) return done(hipfail(ctx, e, "band blocks launch"));
+      if (band_mfma_kernel(lin->Ns, lin->Nq)) {
+        const int BS = b->bw + 1, TI = (BS + 15) / 16, KS = (BS + 3) / 4;
+        const size_t fbytes = (size_t)b->nu * b->KV * 4 * TI * KS * 64 * sizeof(double);
+        if ((e = hipMalloc(&b->frags, fbytes)) != hipSuccess) return done(hipfail(ctx, e, "hipMalloc fragments"));
+        hipLaunchKernelGGL(band_frags_kernel, dim3(b->nu * b->KV), dim3(256), 0, s, g, b->blocks, b->frags, TI, KS);
+        if ((e = hipGetLastError()) != hipSuccess) return done(hipfail(ctx, e, "band fragments launch"));
+      }
+    }
   }
   if ((e = hipStreamSynchronize(s)) != hipSuccess) return done(hipfail(ctx, e, "banded factorisation"));
   rc = done(ATM_OK);
@@ -1440,6 +1871,10 @@ int atm_banded_destroy(atm_banded* b) {
   if (b->line_mat) hipFree(b->line_mat);
   if (b->perm) hipFree(b->perm);
   if (b->wave_mat) hipFree(b->wave_mat);
+  if (b->blocks) hipFree(b->blocks);
+  if (b->frags) hipFree(b->frags);
+  if (b->perm16) hipFree(b->perm16);
+  if (b->wave_mat16) hipFree(b->wave_mat16);
   if (b->elem_col) hipFree(b->elem_col);
   if (b->elem_ez) hipFree(b->elem_ez);
   if (b->colelem) hipFree(b->colelem);
@@ -1452,7 +1887,17 @@ int atm_banded_solve(atm_banded* b, double* x, const double* rhs) {
   atm_ctx* ctx = b->lin->ctx;
   const BandGeo g = band_geo(b);
   const size_t lds = band_solve_lds_bytes(b->n);
-  if (BandSolveRegFn fn = band_reg_kernel(b->lin->Ns, b->lin->Nq)) {
+  BandSolveMfmaFn mfn = b->frags ? band_mfma_kernel(b->lin->Ns, b->lin->Nq) : nullptr;
+  BandSolveBlkFn bfn = b->blocks ? band_blk_kernel(b->lin->Ns, b->lin->Nq) : nullptr;
+  if (mfn && band_mfma_lds_bytes(b->n, b->KV) <= ctx->max_lds) {
+    if (b->nwaves16 > 0)
+      hipLaunchKernelGGL(mfn, dim3(b->nwaves16), dim3(64), band_mfma_lds_bytes(b->n, b->KV), ctx->stream, g,
+                         b->frags, b->perm16, b->wave_mat16, x, rhs);
+  } else if (bfn) {
+    if (b->nwaves16 > 0)
+      hipLaunchKernelGGL(bfn, dim3(b->nwaves16), dim3(64), 0, ctx->stream, g, b->blocks, b->perm16, b->wave_mat16, x,
+                         rhs);
+  } else if (BandSolveRegFn fn = band_reg_kernel(b->lin->Ns, b->lin->Nq)) {
     if (b->nwaves > 0)
       hipLaunchKernelGGL(fn, dim3(b->nwaves), dim3(64), 0, ctx->stream, g, b->fac, b->perm, b->wave_mat, x, rhs);
   } else if (lds <= ctx->max_lds) {
@@ -1529,7 +1974,7 @@ int atm_ark23_run(atm_dg* ex, atm_dg* lin, atm_banded* band, double* q, double* 
           c.ia[k] = ima[i][k];
         }
         hipLaunchKernelGGL(ark_combine_kernel, dim3(blocks), dim3(EW_THREADS), 0, s, Qhat, q, dt, i, c, exK[0],
-                           exK[1], exK[2], imK[0], imK[1], imK[2], n);
+                           exK[1], exK[2], imK[0], imK[1], imK[2], n, split_rhs ? 0 : 1);
         int rc = atm_banded_solve(band, Qs[i], Qhat);
         if (rc) return rc;
       }
@@ -1540,7 +1985,6 @@ int atm_ark23_run(atm_dg* ex, atm_dg* lin, atm_banded* band, double* q, double* 
         hipLaunchKernelGGL(fill_kernel, dim3(blocks), dim3(EW_THREADS), 0, s, exK[i], 0.0, n);
       }
       if ((rc = launch(lin, Qs[i], imK[i], 1.0, 1, nullptr, 0.0))) return rc;
-      if (!split_rhs) hipLaunchKernelGGL(sub_kernel, dim3(blocks), dim3(EW_THREADS), 0, s, exK[i], imK[i], n);
     }
     // q += dt*(b_i exK_i + b~_i imK_i), i = 1..3 (src/odesolvers.jl:382-384)
     ArkCoef c{};
@@ -1549,7 +1993,7 @@ int atm_ark23_run(atm_dg* ex, atm_dg* lin, atm_banded* band, double* q, double* 
       c.ia[k] = exb[k];
     }
     hipLaunchKernelGGL(ark_combine_kernel, dim3(blocks), dim3(EW_THREADS), 0, s, q, q, dt, 3, c, exK[0], exK[1],
-                       exK[2], imK[0], imK[1], imK[2], n);
+                       exK[2], imK[0], imK[1], imK[2], n, split_rhs ? 0 : 1);
     HIPCHK(ctx, hipGetLastError());
   }
   return ATM_OK;

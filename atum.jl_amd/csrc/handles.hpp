@@ -35,6 +35,13 @@ struct atm_dg {
   MeasureFns mfn;
   double* points = nullptr;  // node coordinates (dim blocks of Ntot) when the geometry was built on the device
   double* red = nullptr;  // reduction partials, relaxation scalars, results (allocated on first use)
+  // the last two-step LSRK graph and the arguments it was captured with
+  hipGraphExec_t lsrk_exec = nullptr;
+  int lsrk_scheme = -1;
+  const double* lsrk_q = nullptr;
+  const double* lsrk_dq = nullptr;
+  const double* lsrk_scratch = nullptr;
+  double lsrk_dt = 0.0;
   std::vector<void*> allocs;
 };
 
