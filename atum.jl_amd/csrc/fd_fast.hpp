@@ -103,9 +103,26 @@ struct FastEC {
     P[D + 4] = ib;
   }
 
-  // Symmetric part of F#(a, b) and the two gravity fluctuation scalars.
-  ATM_DEV static void pair(const LawConst& c, double inv_gm1, const double* Pa, const double* Pb,
-                           double F[D][Ns], double& fla, double& flb) {
+  // The symmetric part of F#(a, b) has the form (u = the pair average of the velocity)
+  //   F[d] = [rho_log u_d,  rho_log u_d u + pd e_d,  u_d (rho_log (cc + |u|^2) + pd)],
+  // so its contraction with the metric weights G needs only Gu = G.u:
+  //   G.F = [rho_log Gu,  u (rho_log Gu) + pd G,  Gu (rho_log (cc + |u|^2) + pd)].
+  // pair() returns rho_log, pd, u and the energy coefficient, and the two gravity
+  // fluctuation scalars.
+  struct Pair {
+    double rho_log, pd, ecoef, u[D], fla, flb;
+  };
+  ATM_DEV static void contract(const Pair& w, const double* G, double* gf) {
+#pragma clang fp contract(fast)
+    double Gu = G[0] * w.u[0];
+#pragma unroll
+    for (int d = 1; d < D; ++d) Gu += G[d] * w.u[d];
+    gf[0] = w.rho_log * Gu;
+#pragma unroll
+    for (int e = 0; e < D; ++e) gf[1 + e] = w.u[e] * gf[0] + w.pd * G[e];
+    gf[D + 1] = Gu * w.ecoef;
+  }
+  ATM_DEV static void pair(const LawConst& c, double inv_gm1, const double* Pa, const double* Pb, Pair& w) {
 #pragma clang fp contract(fast)
     const double ra = Pa[0], rb = Pb[0], ba = Pa[D + 1], bb = Pb[D + 1];
     const double sr = ra + rb, sb = ba + bb;
@@ -123,32 +140,24 @@ struct FastEC {
     const double rho_log = 0.5 * sr * Gr;
     const double pd = 0.5 * sr * isb;              // rho_avg / (2 b_avg)
     const double inv2bl = Fb * isb * inv_gm1;      // 1 / (2 (gamma-1) b_log)
-    double u_avg[D];
-#pragma unroll
-    for (int d = 0; d < D; ++d) u_avg[d] = 0.5 * (Pa[1 + d] + Pb[1 + d]);
-    double cc = inv2bl - 0.25 * (Pa[D + 2] + Pb[D + 2]);
-    if (GRAV) cc += 0.5 * (Pa[D + 3] + Pb[D + 3]);
+    double uu = 0;
 #pragma unroll
     for (int d = 0; d < D; ++d) {
-      const double frd = u_avg[d] * rho_log;
-      F[d][0] = frd;
-      double mv = 0;
-#pragma unroll
-      for (int e = 0; e < D; ++e) {
-        double fu = u_avg[e] * frd;  // u_avg[d]*frho[e] with frho[e] = u_avg[e]*rho_log
-        if (d == e) fu += pd;
-        F[d][1 + e] = fu;
-        mv += fu * u_avg[e];
-      }
-      F[d][D + 1] = cc * frd + mv;
+      w.u[d] = 0.5 * (Pa[1 + d] + Pb[1 + d]);
+      uu += w.u[d] * w.u[d];
     }
+    double cc = inv2bl - 0.25 * (Pa[D + 2] + Pb[D + 2]);
+    if (GRAV) cc += 0.5 * (Pa[D + 3] + Pb[D + 3]);
+    w.rho_log = rho_log;
+    w.pd = pd;
+    w.ecoef = rho_log * (cc + uu) + pd;
     if (GRAV) {
-      const double w = 0.25 * sb * rho_log;  // b_avg rho_log / 2
+      const double wg = 0.25 * sb * rho_log;  // b_avg rho_log / 2
       const double dphi = Pa[D + 3] - Pb[D + 3];
-      fla = w * Pa[D + 4] * dphi;
-      flb = -w * Pb[D + 4] * dphi;
+      w.fla = wg * Pa[D + 4] * dphi;
+      w.flb = -wg * Pb[D + 4] * dphi;
     } else {
-      fla = flb = 0.0;
+      w.fla = w.flb = 0.0;
     }
   }
 };
@@ -226,29 +235,24 @@ ATM_DEV void fd_fast_body(const DevDG& p, double* lds, int64_t blk, const double
         double P2[NP];
 #pragma unroll
         for (int k = 0; k < NP; ++k) P2[k] = sP[k * Np + ib];
-        double F[D][Ns], fla, flb;
-        FE::pair(p.c, inv_gm1, P1, P2, F, fla, flb);
+        typename FE::Pair w;
+        FE::pair(p.c, inv_gm1, P1, P2, w);
         const double Dba = sD[b + Nq * a], Dab = sD[a + Nq * b];
         double G[D];
 #pragma unroll
         for (int d = 0; d < D; ++d) G[d] = Dba * slg[d * Np + ib] - Dab * lga[d];
         double gf[Ns];
-#pragma unroll
-        for (int s = 0; s < Ns; ++s) {
-          double t = G[0] * F[0][s];
-#pragma unroll
-          for (int d = 1; d < D; ++d) t += G[d] * F[d][s];
-          gf[s] = t;
-        }
+        FE::contract(w, G, gf);
 #pragma unroll
         for (int s = 0; s < Ns; ++s) vol[s] += gf[s];
 #pragma unroll
-        for (int d = 0; d < D; ++d) vol[1 + d] -= G[d] * fla;
+        for (int d = 0; d < D; ++d) vol[1 + d] -= G[d] * w.fla;
+        // the partner's share: -G.F, plus its own gravity fluctuation term
         double* slot = sx + (m - 1) * Ns * Np + ib;
+        slot[0] = -gf[0];
 #pragma unroll
-        for (int s = 0; s < Ns; ++s) slot[s * Np] = -gf[s];
-#pragma unroll
-        for (int d = 0; d < D; ++d) slot[(1 + d) * Np] += G[d] * flb;
+        for (int d = 0; d < D; ++d) slot[(1 + d) * Np] = G[d] * w.flb - gf[1 + d];
+        slot[(D + 1) * Np] = -gf[D + 1];
       }
     }
     __syncthreads();
