@@ -69,6 +69,10 @@ _sig("atm_ctx_create", c_int, [c_int, ctypes.POINTER(c_void_p)])
 _sig("atm_ctx_destroy", c_int, [c_void_p])
 _sig("atm_ctx_set_stream", c_int, [c_void_p, c_void_p])
 _sig("atm_ctx_get_stream", c_void_p, [c_void_p])
+_sig("atm_stream_create", c_int, [c_void_p, ctypes.POINTER(c_void_p)])
+_sig("atm_stream_destroy", c_int, [c_void_p, c_void_p])
+_sig("atm_stream_sync", c_int, [c_void_p, c_void_p])
+_sig("atm_stream_wait", c_int, [c_void_p, c_void_p, c_void_p])
 _sig("atm_last_error", ctypes.c_char_p, [c_void_p])
 _sig("atm_sync", c_int, [c_void_p])
 _sig("atm_version", ctypes.c_char_p, [])
@@ -135,6 +139,24 @@ class Context:
     @property
     def stream(self):
         return lib.atm_ctx_get_stream(self.h)
+
+    def new_stream(self):
+        """A further HIP stream on this device (default priority); see `use_stream`."""
+        h = c_void_p()
+        self.check(lib.atm_stream_create(self.h, ctypes.byref(h)), "atm_stream_create")
+        return h
+
+    def destroy_stream(self, stream):
+        self.check(lib.atm_stream_destroy(self.h, stream), "atm_stream_destroy")
+
+    def stream_wait(self, waiter=None, waitee=None):
+        """Order `waiter` after the work already enqueued on `waitee` (None: the
+        context's current stream)."""
+        self.check(lib.atm_stream_wait(self.h, waiter, waitee), "atm_stream_wait")
+
+    def use_stream(self, stream=None):
+        """Launch on `stream` from now on (None: the context's own stream)."""
+        self.check(lib.atm_ctx_set_stream(self.h, stream), "atm_ctx_set_stream")
 
     def record(self, slot):
         self.check(lib.atm_event_record(self.h, slot), "atm_event_record")

@@ -3,8 +3,11 @@
 Continuous batching: every tick admits the requests that have arrived (while
 the device still holds fewer than `max_active` requests and their HBM footprint
 fits the budget), advances every active request by a common chunk of steps —
-all explicit (LSRK) requests in one batched launch per kernel group and stage,
-implicit-explicit (ARK23) requests on the same stream right after — and retires
+all explicit (LSRK) requests in one batched launch per kernel group and stage
+on the context's stream, implicit-explicit (ARK23) requests spread over
+`imex_streams` further streams that run beside it (each request's steps in
+order on one stream; the streams fork from and join back into the context's
+stream every tick) — and retires
 the requests that finished, freeing their slots and memory for the next tick.
 A request's state stays resident in HBM from admission to completion, and its
 result is bitwise the one it would have alone (same kernels, same order).
@@ -76,8 +79,11 @@ class ServeStats:
 
 
 class Server:
-    def __init__(self, ctx=None, hbm_budget=0.9 * 288e9, max_active=512, chunk=10, keep_results=True):
+    def __init__(self, ctx=None, hbm_budget=0.9 * 288e9, max_active=512, chunk=10, keep_results=True,
+                 imex_streams=3):
         self.ctx = ctx or _lib.context()
+        self.imex_streams = int(imex_streams)
+        self._streams = []
         self.hbm_budget = float(hbm_budget)
         self.max_active = int(max_active)
         self.chunk = int(chunk)
@@ -118,6 +124,40 @@ class Server:
         self.in_use -= r.footprint
         r._q = r._dq = r._scratch = r._ark = r._dg = None
 
+    def _run_implicit(self, implicit, step):
+        """ARK23 chunks of the implicit requests, round-robin over the side streams
+        (or on the context's stream when imex_streams is 0)."""
+        if self.imex_streams <= 0:
+            for r in implicit:
+                r._ark.run(r._q, step)
+            return
+        while len(self._streams) < self.imex_streams:
+            self._streams.append(self.ctx.new_stream())
+        main = self.ctx.stream
+        used = self._streams[:min(len(implicit), self.imex_streams)]
+        for s in used:
+            self.ctx.stream_wait(s, main)
+        try:
+            for i, r in enumerate(implicit):
+                self.ctx.use_stream(used[i % len(used)])
+                r._ark.run(r._q, step)
+        finally:
+            self.ctx.use_stream(main)
+        for s in used:
+            self.ctx.stream_wait(main, s)
+
+    def close(self):
+        """Release the side streams (after their work has finished)."""
+        for s in self._streams:
+            self.ctx.destroy_stream(s)
+        self._streams = []
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
     def run(self, requests, on_tick=None):
         """Serve `requests`; a request becomes available when the wall clock since the
         start of the run passes its arrival time."""
@@ -153,8 +193,8 @@ class Server:
                     members = ids
                 batch.lsrk_run([r._q for r in explicit], [r._dq for r in explicit],
                                [r._scratch for r in explicit], [r._dt for r in explicit], step, LSRK54)
-            for r in implicit:
-                r._ark.run(r._q, step)
+            if implicit:
+                self._run_implicit(implicit, step)
             self.ctx.record(15)
             tw = time.perf_counter()
             stats.enqueue_s += tw - te

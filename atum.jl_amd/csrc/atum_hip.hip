@@ -1085,6 +1085,44 @@ int atm_ctx_destroy(atm_ctx* ctx) {
   return ATM_OK;
 }
 
+int atm_stream_create(atm_ctx* ctx, void** stream) {
+  if (!ctx || !stream) return ATM_ERR_ARG;
+  hipSetDevice(ctx->device);
+  hipStream_t s = nullptr;
+  HIPCHK(ctx, hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+  *stream = (void*)s;
+  return ATM_OK;
+}
+
+int atm_stream_destroy(atm_ctx* ctx, void* stream) {
+  if (!ctx || !stream) return ATM_ERR_ARG;
+  hipSetDevice(ctx->device);
+  HIPCHK(ctx, hipStreamSynchronize((hipStream_t)stream));
+  HIPCHK(ctx, hipStreamDestroy((hipStream_t)stream));
+  return ATM_OK;
+}
+
+int atm_stream_sync(atm_ctx* ctx, void* stream) {
+  if (!ctx || !stream) return ATM_ERR_ARG;
+  HIPCHK(ctx, hipStreamSynchronize((hipStream_t)stream));
+  return ATM_OK;
+}
+
+int atm_stream_wait(atm_ctx* ctx, void* waiter, void* waitee) {
+  if (!ctx) return ATM_ERR_ARG;
+  hipStream_t a = waiter ? (hipStream_t)waiter : ctx->stream;
+  hipStream_t b = waitee ? (hipStream_t)waitee : ctx->stream;
+  if (a == b) return ATM_OK;
+  hipSetDevice(ctx->device);
+  hipEvent_t ev = nullptr;
+  HIPCHK(ctx, hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+  hipError_t e = hipEventRecord(ev, b);
+  if (e == hipSuccess) e = hipStreamWaitEvent(a, ev, 0);
+  hipEventDestroy(ev);  // released once the wait has been satisfied
+  if (e != hipSuccess) return hipfail(ctx, e, "atm_stream_wait");
+  return ATM_OK;
+}
+
 int atm_ctx_set_stream(atm_ctx* ctx, void* hip_stream) {
   if (!ctx) return ATM_ERR_ARG;
   ctx->stream = hip_stream ? static_cast<hipStream_t>(hip_stream) : ctx->own;

@@ -240,7 +240,7 @@ def serve_workload(args):
     from serve_requests import make_stream
 
     reqs = make_stream(duration=args.serve_seconds)
-    server = A.Server(chunk=args.chunk, keep_results=False)
+    server = A.Server(chunk=args.chunk, keep_results=False, imex_streams=args.imex_streams)
     stats = server.run(reqs)
     s = stats.summary()
     kinds = {k: sum(r.kind == k for r in reqs) for k in ("vortex", "sw", "imex")}
@@ -250,7 +250,7 @@ def serve_workload(args):
             "vs_baseline": None, "dtype": "f64", "data": "synthetic",
             "config": {"workload": f"Poisson 50/s for {args.serve_seconds} s, mix 50/30/20 "
                                    "vortex/shallow-water/IMEX, steps U{50..500}", "requests": len(reqs),
-                       "kinds": kinds, "chunk_steps": args.chunk},
+                       "kinds": kinds, "chunk_steps": args.chunk, "imex_streams": args.imex_streams},
             "p50_s": s["p50_s"], "p99_s": s["p99_s"], "mean_s": s["mean_s"], "completed": s["completed"],
             "wall_s": s["wall_s"], "ticks": s["ticks"], "max_active": s["max_active"],
             "breakdown_s": {k: s[k] for k in ("admit_s", "enqueue_s", "wait_s", "retire_s", "gpu_span_s")}}
@@ -303,6 +303,8 @@ def main():
     ap.add_argument("--instances", type=int, default=256)
     ap.add_argument("--serve-seconds", type=float, default=10.0)
     ap.add_argument("--chunk", type=int, default=10, help="steps per serving tick")
+    ap.add_argument("--imex-streams", type=int, default=3,
+                    help="side streams for the IMEX requests of the served stream (0: the main stream)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build measures one GPU; the path shards by elements with one face "

@@ -128,6 +128,15 @@ int atm_ctx_destroy(atm_ctx* ctx);
 int atm_ctx_set_stream(atm_ctx* ctx, void* hip_stream);
 /* the hipStream_t the context launches on (for the caller's event timing) */
 void* atm_ctx_get_stream(atm_ctx* ctx);
+/* Additional streams on the context's device (default priority, non-blocking),
+ * e.g. to run independent operators concurrently by switching the context's
+ * stream with atm_ctx_set_stream; destroy waits for the stream's work. */
+int atm_stream_create(atm_ctx* ctx, void** stream);
+int atm_stream_destroy(atm_ctx* ctx, void* stream);
+int atm_stream_sync(atm_ctx* ctx, void* stream);
+/* Work enqueued on `waiter` after this call starts after the work enqueued on
+ * `waitee` before it (an event, no host wait); NULL names the context's stream. */
+int atm_stream_wait(atm_ctx* ctx, void* waiter, void* waitee);
 const char* atm_last_error(atm_ctx* ctx);
 int atm_sync(atm_ctx* ctx);
 const char* atm_version(void);
