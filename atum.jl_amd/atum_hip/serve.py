@@ -1,21 +1,20 @@
 """Serving loop: simulation requests arrive over time and share the device.
 
-Continuous batching: every tick admits the requests that have arrived (while
-the device still holds fewer than `max_active` requests and their HBM footprint
-fits the budget), advances every active request by a common chunk of steps —
-all explicit (LSRK) requests in one batched launch per kernel group and stage
-on the context's stream, implicit-explicit (ARK23) requests spread over
-`imex_streams` further streams that run beside it (each request's steps in
-order on one stream; the streams fork from and join back into the context's
-stream every tick) — and retires
-the requests that finished, freeing their slots and memory for the next tick.
-A request's state stays resident in HBM from admission to completion, and its
-result is bitwise the one it would have alone (same kernels, same order).
-Operators (geometry on the device) are cached per mesh, so repeated meshes
-cost one upload.
+`Server` drives the library's serving scheduler (atm_server_*): requests are
+built (operators cached per mesh, so repeated meshes cost one upload) and
+submitted from this thread as they arrive, while the library's worker thread
+admits them (while the device holds fewer than `max_active` requests and
+their HBM footprint fits the budget), advances every resident request by a
+common chunk of steps per tick — the explicit (LSRK) requests in one batched
+launch per kernel group and stage, the implicit-explicit (ARK23) requests on
+`imex_streams` side streams beside it — and retires the finished ones,
+copying their final state to the host.  Building the next requests on the host
+therefore overlaps the device work.  A request's result is bitwise the one it
+would have alone (same kernels, same order).
 """
 from __future__ import annotations
 
+import ctypes
 import math
 import time
 from dataclasses import dataclass, field
@@ -24,9 +23,46 @@ from typing import Callable, Optional
 import numpy as np
 
 from . import _lib
-from .batch import Batch
-from .fields import Field
 from .odesolvers import ARK23, LSRK54, BandedOperator
+
+c_int32, c_int64, c_double, c_void_p = ctypes.c_int32, ctypes.c_int64, ctypes.c_double, ctypes.c_void_p
+
+REQ_LSRK, REQ_ARK23 = 0, 1
+QUEUED, ACTIVE, DONE = 0, 1, 2
+
+
+class _Desc(ctypes.Structure):
+    _fields_ = [("max_active", c_int32), ("chunk", c_int32), ("imex_streams", c_int32), ("reserved", c_int32),
+                ("hbm_budget", c_double)]
+
+
+class _Req(ctypes.Structure):
+    _fields_ = [("kind", c_int32), ("scheme", c_int32), ("split_rhs", c_int32), ("paperversion", c_int32),
+                ("dg", c_void_p), ("lin", c_void_p), ("band", c_void_p), ("q0", c_void_p), ("result", c_void_p),
+                ("dt", c_double), ("steps", c_int64)]
+
+
+_STAT_INT = ("submitted", "completed", "failed", "ticks", "max_active")
+_STAT_DBL = ("dof_updates", "busy_s", "gpu_span_s", "admit_s", "enqueue_s", "wait_s", "retire_s")
+
+
+class _Stats(ctypes.Structure):
+    _fields_ = [(n, c_int64) for n in _STAT_INT] + [(n, c_double) for n in _STAT_DBL]
+
+
+_P = ctypes.POINTER
+_lib._sig("atm_server_create", ctypes.c_int, [c_void_p, _P(_Desc), _P(c_void_p)])
+_lib._sig("atm_server_destroy", ctypes.c_int, [c_void_p])
+_lib._sig("atm_server_submit", ctypes.c_int, [c_void_p, _P(_Req), _P(c_int64)])
+_lib._sig("atm_server_poll", ctypes.c_int, [c_void_p, c_int64])
+_lib._sig("atm_server_wait", ctypes.c_int, [c_void_p, c_int64])
+_lib._sig("atm_server_next_done", ctypes.c_int, [c_void_p, c_double, _P(c_int64)])
+_lib._sig("atm_server_drain", ctypes.c_int, [c_void_p])
+_lib._sig("atm_server_clock", c_double, [c_void_p])
+_lib._sig("atm_server_times", ctypes.c_int, [c_void_p, c_int64, _P(c_double)])
+_lib._sig("atm_server_error", ctypes.c_char_p, [c_void_p, c_int64])
+_lib._sig("atm_server_release", ctypes.c_int, [c_void_p, c_int64])
+_lib._sig("atm_server_get_stats", ctypes.c_int, [c_void_p, _P(_Stats)])
 
 
 @dataclass
@@ -43,179 +79,168 @@ class Request:
     admitted: Optional[float] = None
     finished: Optional[float] = None
     result: Optional[np.ndarray] = None
+    error: Optional[str] = None
     dof: int = 0
-    # device-side state while active
-    _dg: object = None
-    _q: Optional[Field] = None
-    _dq: Optional[Field] = None
-    _scratch: Optional[Field] = None
-    _dt: float = 0.0
-    _ark: object = None
-    _left: int = 0
+    _id: int = -1
+    _keep: object = None  # host buffers and operators the library reads until the request ends
 
 
 @dataclass
 class ServeStats:
     completed: int = 0
+    failed: int = 0
     latencies: list = field(default_factory=list)
     dof_updates: float = 0.0
     wall: float = 0.0
     ticks: int = 0
     max_active: int = 0
-    admit_s: float = 0.0     # host: building / uploading admitted requests
-    enqueue_s: float = 0.0   # host: launching a tick's work
-    wait_s: float = 0.0      # host: waiting for the device at the end of a tick
-    retire_s: float = 0.0    # host: downloading results, freeing
+    build_s: float = 0.0     # host (caller thread): building and submitting requests
+    admit_s: float = 0.0     # worker: allocating and uploading admitted requests
+    enqueue_s: float = 0.0   # worker: launching the ticks
+    wait_s: float = 0.0      # worker: waiting for the device
+    retire_s: float = 0.0    # worker: downloading results, freeing
     gpu_span_s: float = 0.0  # device: from the first to the last kernel of each tick (events)
 
     def summary(self):
         lat = np.array(self.latencies) if self.latencies else np.zeros(1)
-        return {"completed": self.completed, "p50_s": float(np.percentile(lat, 50)),
+        return {"completed": self.completed, "failed": self.failed, "p50_s": float(np.percentile(lat, 50)),
                 "p99_s": float(np.percentile(lat, 99)), "mean_s": float(lat.mean()),
                 "dof_updates_per_s": self.dof_updates / self.wall if self.wall > 0 else 0.0,
                 "wall_s": self.wall, "ticks": self.ticks, "max_active": self.max_active,
-                "admit_s": self.admit_s, "enqueue_s": self.enqueue_s, "wait_s": self.wait_s,
-                "retire_s": self.retire_s, "gpu_span_s": self.gpu_span_s}
+                "build_s": self.build_s, "admit_s": self.admit_s, "enqueue_s": self.enqueue_s,
+                "wait_s": self.wait_s, "retire_s": self.retire_s, "gpu_span_s": self.gpu_span_s}
 
 
 class Server:
     def __init__(self, ctx=None, hbm_budget=0.9 * 288e9, max_active=512, chunk=10, keep_results=True,
                  imex_streams=3):
         self.ctx = ctx or _lib.context()
-        self.imex_streams = int(imex_streams)
-        self._streams = []
-        self.hbm_budget = float(hbm_budget)
-        self.max_active = int(max_active)
-        self.chunk = int(chunk)
         self.keep_results = keep_results
+        self.imex_streams = int(imex_streams)
         self.cache = {}
-        self.in_use = 0
+        self._live = {}
+        desc = _Desc(int(max_active), int(chunk), self.imex_streams, 0, float(hbm_budget))
+        h = c_void_p()
+        self.ctx.check(_lib.lib.atm_server_create(self.ctx.h, ctypes.byref(desc), ctypes.byref(h)),
+                       "atm_server_create")
+        self.h = h
 
-    def _admit(self, r: Request, now):
+    def submit(self, r: Request) -> int:
+        """Build the request's operators (cached per mesh) and hand it to the scheduler."""
         built = r.build(self.cache)
+        req = _Req()
         if r.kind == "imex":
             ex, lin, q0, dt = built
-            r._dg = lin
-            r._q = lin.fieldarray(q0)
             # requests on the same mesh with the same time step share one factorisation
             bkey = ("banded", r.key, dt)
             if bkey not in self.cache:
                 self.cache[bkey] = BandedOperator(lin, -dt * (1 - 1 / math.sqrt(2)))
-            r._ark = ARK23(ex, lin, r._q, dt, split_rhs=False, banded=self.cache[bkey])
+            band = self.cache[bkey]
+            req.kind, req.dg, req.lin, req.band = REQ_ARK23, (ex.h if ex is not None else None), lin.h, band.h
+            req.split_rhs, req.paperversion = 0, 0
+            shape_dg = lin
         else:
             dg, q0, dt = built
-            r._dg = dg
-            r._q = dg.fieldarray(q0)
-            r._dq = r._q.similar().fill_(0.0)
-            r._scratch = r._q.similar()
-        r._dt = dt
+            band = None
+            req.kind, req.scheme, req.dg = REQ_LSRK, 0, dg.h
+            shape_dg = dg
+        q0 = np.ascontiguousarray(q0, dtype=np.float64)
+        ns, nn = int(_lib.lib.atm_dg_nstates(shape_dg.h)), int(_lib.lib.atm_dg_nnodes(shape_dg.h))
+        if q0.size != ns * nn:
+            raise ValueError(f"request {r.rid}: initial state has {q0.size} values, the operator {ns * nn}")
+        r.result = np.empty((ns, nn)) if self.keep_results else None
+        req.q0 = q0.ctypes.data
+        req.result = r.result.ctypes.data if r.result is not None else None
+        req.dt, req.steps = float(dt), int(r.steps)
+        r._keep = (built, band, q0)
         r.dof = int(q0.size)
-        r._left = r.steps
-        r.admitted = now
-        self.in_use += r.footprint
+        i = c_int64()
+        self.ctx.check(_lib.lib.atm_server_submit(self.h, ctypes.byref(req), ctypes.byref(i)), "atm_server_submit")
+        r._id = int(i.value)
+        self._live[r._id] = r
+        return r._id
 
-    def _retire(self, r: Request, now, stats: ServeStats):
-        self.ctx.sync()
-        if self.keep_results:
-            r.result = r._q.numpy()
-        r.finished = now
-        stats.completed += 1
-        stats.latencies.append(r.finished - r.arrival)
-        self.in_use -= r.footprint
-        r._q = r._dq = r._scratch = r._ark = r._dg = None
+    def _finish(self, i: int) -> Request:
+        r = self._live.pop(i)
+        st = _lib.lib.atm_server_poll(self.h, i)
+        t = (c_double * 3)()
+        _lib.lib.atm_server_times(self.h, i, t)
+        r.admitted, r.finished = t[1], t[2]
+        if st < 0:
+            r.error = (_lib.lib.atm_server_error(self.h, i) or b"").decode()
+            r.result = None
+        _lib.lib.atm_server_release(self.h, i)
+        r._keep = None
+        return r
 
-    def _run_implicit(self, implicit, step):
-        """ARK23 chunks of the implicit requests, round-robin over the side streams
-        (or on the context's stream when imex_streams is 0)."""
-        if self.imex_streams <= 0:
-            for r in implicit:
-                r._ark.run(r._q, step)
-            return
-        while len(self._streams) < self.imex_streams:
-            self._streams.append(self.ctx.new_stream())
-        main = self.ctx.stream
-        used = self._streams[:min(len(implicit), self.imex_streams)]
-        for s in used:
-            self.ctx.stream_wait(s, main)
-        try:
-            for i, r in enumerate(implicit):
-                self.ctx.use_stream(used[i % len(used)])
-                r._ark.run(r._q, step)
-        finally:
-            self.ctx.use_stream(main)
-        for s in used:
-            self.ctx.stream_wait(main, s)
+    def next_done(self, timeout: float = -1.0) -> Optional[Request]:
+        """The next finished (or failed) request, or None after `timeout` seconds (< 0: wait)."""
+        i = c_int64()
+        if _lib.lib.atm_server_next_done(self.h, float(timeout), ctypes.byref(i)) != 1:
+            return None
+        return self._finish(int(i.value))
+
+    def clock(self) -> float:
+        """Seconds on the scheduler's clock (request times use it)."""
+        return _lib.lib.atm_server_clock(self.h)
+
+    def stats(self) -> dict:
+        st = _Stats()
+        _lib.lib.atm_server_get_stats(self.h, ctypes.byref(st))
+        return {n: getattr(st, n) for n in _STAT_INT + _STAT_DBL}
+
+    def run(self, requests):
+        """Serve `requests`: a request is submitted when the wall clock since the start of
+        the run passes its arrival time; returns when all have finished.  Raises
+        AtumError if any request failed (after the others have finished)."""
+        pending = sorted(requests, key=lambda r: r.arrival)
+        before = self.stats()
+        stats = ServeStats()
+        t0 = time.perf_counter()
+        c0 = self.clock()
+        pi, waiting, failed = 0, 0, []
+        while pi < len(pending) or waiting:
+            tb = time.perf_counter()
+            while pi < len(pending) and pending[pi].arrival <= tb - t0:
+                self.submit(pending[pi])
+                pi += 1
+                waiting += 1
+            stats.build_s += time.perf_counter() - tb
+            timeout = 0.05
+            if pi < len(pending):
+                timeout = max(0.0, min(timeout, pending[pi].arrival - (time.perf_counter() - t0)))
+            r = self.next_done(timeout)
+            while r is not None:
+                waiting -= 1
+                if r.error is not None:
+                    failed.append(r)
+                else:
+                    stats.latencies.append(r.finished - (c0 + r.arrival))
+                r = self.next_done(0.0)
+        stats.wall = time.perf_counter() - t0
+        after = self.stats()
+        stats.completed = after["completed"] - before["completed"]
+        stats.failed = after["failed"] - before["failed"]
+        stats.ticks = after["ticks"] - before["ticks"]
+        stats.max_active = after["max_active"]
+        stats.dof_updates = after["dof_updates"] - before["dof_updates"]
+        for k in ("admit_s", "enqueue_s", "wait_s", "retire_s", "gpu_span_s"):
+            setattr(stats, k, after[k] - before[k])
+        if failed:
+            raise _lib.AtumError(f"{len(failed)} request(s) failed; request {failed[0].rid}: {failed[0].error}")
+        return stats
 
     def close(self):
-        """Release the side streams (after their work has finished)."""
-        for s in self._streams:
-            self.ctx.destroy_stream(s)
-        self._streams = []
+        """Finish every submitted request and stop the scheduler."""
+        if getattr(self, "h", None):
+            _lib.lib.atm_server_destroy(self.h)
+            self.h = None
 
     def __del__(self):
         try:
             self.close()
         except Exception:
             pass
-
-    def run(self, requests, on_tick=None):
-        """Serve `requests`; a request becomes available when the wall clock since the
-        start of the run passes its arrival time."""
-        pending = sorted(requests, key=lambda r: r.arrival)
-        active = []
-        stats = ServeStats()
-        batch, members = None, None
-        t0 = time.perf_counter()
-        pi = 0
-        while pi < len(pending) or active:
-            now = time.perf_counter() - t0
-            ta = time.perf_counter()
-            while (pi < len(pending) and pending[pi].arrival <= now and len(active) < self.max_active
-                   and self.in_use + pending[pi].footprint <= self.hbm_budget):
-                self._admit(pending[pi], time.perf_counter() - t0)
-                active.append(pending[pi])
-                pi += 1
-            stats.admit_s += time.perf_counter() - ta
-            if not active:
-                if pi < len(pending):
-                    time.sleep(max(0.0, min(0.01, pending[pi].arrival - now)))
-                continue
-            stats.max_active = max(stats.max_active, len(active))
-            explicit = [r for r in active if r.kind != "imex"]
-            implicit = [r for r in active if r.kind == "imex"]
-            step = min(self.chunk, min(r._left for r in active))
-            te = time.perf_counter()
-            self.ctx.record(14)
-            if explicit:
-                ids = [r.rid for r in explicit]
-                if ids != members:
-                    batch = Batch([r._dg for r in explicit], self.ctx)
-                    members = ids
-                batch.lsrk_run([r._q for r in explicit], [r._dq for r in explicit],
-                               [r._scratch for r in explicit], [r._dt for r in explicit], step, LSRK54)
-            if implicit:
-                self._run_implicit(implicit, step)
-            self.ctx.record(15)
-            tw = time.perf_counter()
-            stats.enqueue_s += tw - te
-            self.ctx.sync()
-            stats.wait_s += time.perf_counter() - tw
-            stats.gpu_span_s += self.ctx.elapsed_ms(14, 15) / 1e3
-            stats.ticks += 1
-            for r in active:
-                r._left -= step
-                stats.dof_updates += r.dof * step * (3 if r.kind == "imex" else 5)
-            now = time.perf_counter() - t0
-            done = [r for r in active if r._left == 0]
-            tr = time.perf_counter()
-            for r in done:
-                self._retire(r, now, stats)
-            stats.retire_s += time.perf_counter() - tr
-            active = [r for r in active if r._left > 0]
-            if on_tick:
-                on_tick(stats, active)
-        stats.wall = time.perf_counter() - t0
-        return stats
 
 
 def run_alone(r: Request, cache=None):

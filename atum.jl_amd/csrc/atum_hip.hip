@@ -51,6 +51,11 @@ struct atm_banded {
   int32_t* colelem = nullptr;   // ncols*KV
 };
 
+namespace atm {
+thread_local hipStream_t tl_stream = nullptr;
+atm_dg* banded_lin(const atm_banded* b) { return b ? b->lin : nullptr; }
+}
+
 namespace {
 
 constexpr int EW_THREADS = 256;
@@ -1027,7 +1032,7 @@ int reduce_to_host(atm_dg* dg, int mode, const double* q, const double* k, doubl
   if (!dg->mfn.reduce) return fail(ctx, ATM_ERR_UNSUPPORTED, "the balance law defines no entropy");
   int rc = red_buffer(dg);
   if (rc) return rc;
-  hipStream_t s = ctx->stream;
+  hipStream_t s = stream_of(ctx);
   HIPCHK(ctx, dg->mfn.reduce(dg->dev, s, mode, q, k, nullptr, red_state(dg), dg->red));
   HIPCHK(ctx, launch_finalize(s, dg->red, red_blocks(dg->Ntot), 1, FIN_OUT, 0.0, red_state(dg), red_out(dg)));
   HIPCHK(ctx, hipMemcpyAsync(host, red_out(dg), sizeof(double), hipMemcpyDeviceToHost, s));
@@ -1036,7 +1041,7 @@ int reduce_to_host(atm_dg* dg, int mode, const double* q, const double* k, doubl
 }
 
 int launch(atm_dg* dg, const double* q, double* dq, double alpha, int assign, double* qout, double bdt) {
-  hipError_t e = dg->fn(dg->dev, dg->ctx->stream, q, dq, alpha, assign, qout, bdt);
+  hipError_t e = dg->fn(dg->dev, stream_of(dg->ctx), q, dq, alpha, assign, qout, bdt);
   if (e != hipSuccess) return hipfail(dg->ctx, e, "element kernel launch");
   return ATM_OK;
 }
@@ -1110,8 +1115,8 @@ int atm_stream_sync(atm_ctx* ctx, void* stream) {
 
 int atm_stream_wait(atm_ctx* ctx, void* waiter, void* waitee) {
   if (!ctx) return ATM_ERR_ARG;
-  hipStream_t a = waiter ? (hipStream_t)waiter : ctx->stream;
-  hipStream_t b = waitee ? (hipStream_t)waitee : ctx->stream;
+  hipStream_t a = waiter ? (hipStream_t)waiter : stream_of(ctx);
+  hipStream_t b = waitee ? (hipStream_t)waitee : stream_of(ctx);
   if (a == b) return ATM_OK;
   hipSetDevice(ctx->device);
   hipEvent_t ev = nullptr;
@@ -1135,7 +1140,7 @@ const char* atm_last_error(atm_ctx* ctx) { return ctx ? ctx->err.c_str() : "null
 
 int atm_sync(atm_ctx* ctx) {
   if (!ctx) return ATM_ERR_ARG;
-  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  HIPCHK(ctx, hipStreamSynchronize(stream_of(ctx)));
   return ATM_OK;
 }
 
@@ -1156,21 +1161,21 @@ int atm_free(atm_ctx* ctx, void* dptr) {
 
 int atm_upload(atm_ctx* ctx, void* dst, const void* src, size_t bytes) {
   if (!ctx) return ATM_ERR_ARG;
-  HIPCHK(ctx, hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, ctx->stream));
-  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  HIPCHK(ctx, hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, stream_of(ctx)));
+  HIPCHK(ctx, hipStreamSynchronize(stream_of(ctx)));
   return ATM_OK;
 }
 
 int atm_download(atm_ctx* ctx, void* dst, const void* src, size_t bytes) {
   if (!ctx) return ATM_ERR_ARG;
-  HIPCHK(ctx, hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, ctx->stream));
-  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  HIPCHK(ctx, hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, stream_of(ctx)));
+  HIPCHK(ctx, hipStreamSynchronize(stream_of(ctx)));
   return ATM_OK;
 }
 
 int atm_copy(atm_ctx* ctx, void* dst, const void* src, size_t bytes) {
   if (!ctx) return ATM_ERR_ARG;
-  HIPCHK(ctx, hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, ctx->stream));
+  HIPCHK(ctx, hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, stream_of(ctx)));
   return ATM_OK;
 }
 
@@ -1324,7 +1329,7 @@ int atm_dg_create(atm_ctx* ctx, const atm_dg_desc* d, atm_dg** out) {
 int atm_dg_destroy(atm_dg* dg) {
   if (!dg) return ATM_ERR_ARG;
   hipSetDevice(dg->ctx->device);
-  hipStreamSynchronize(dg->ctx->stream);
+  hipStreamSynchronize(stream_of(dg->ctx));
   if (dg->lsrk_exec) hipGraphExecDestroy(dg->lsrk_exec);
   for (void* p : dg->allocs) hipFree(p);
   delete dg;
@@ -1374,7 +1379,7 @@ int atm_lsrk_run(atm_dg* dg, int scheme, double* q, double* dq, double* scratch,
   int64_t done = 0;
   if (use_graph && nsteps >= pair) {
     // reuse the last instantiated graph when the buffers, dt and scheme are the same
-    hipStream_t s = dg->ctx->stream;
+    hipStream_t s = stream_of(dg->ctx);
     if (dg->lsrk_exec && (dg->lsrk_scheme != scheme || dg->lsrk_q != q || dg->lsrk_dq != dq ||
                           dg->lsrk_scratch != scratch || dg->lsrk_dt != dt)) {
       HIPCHK(dg->ctx, hipStreamSynchronize(s));
@@ -1413,7 +1418,7 @@ int atm_lsrk_run(atm_dg* dg, int scheme, double* q, double* dq, double* scratch,
   }
   if (cur != 0) {
     const int64_t n = (int64_t)dg->Ns * dg->Ntot;
-    hipLaunchKernelGGL(copy_kernel, dim3(ew_blocks(n)), dim3(EW_THREADS), 0, dg->ctx->stream, q, scratch, n);
+    hipLaunchKernelGGL(copy_kernel, dim3(ew_blocks(n)), dim3(EW_THREADS), 0, stream_of(dg->ctx), q, scratch, n);
     HIPCHK(dg->ctx, hipGetLastError());
   }
   return ATM_OK;
@@ -1422,7 +1427,7 @@ int atm_lsrk_run(atm_dg* dg, int scheme, double* q, double* dq, double* scratch,
 int atm_event_record(atm_ctx* ctx, int slot) {
   if (!ctx || slot < 0 || slot >= 16) return ATM_ERR_ARG;
   if (!ctx->events[slot]) HIPCHK(ctx, hipEventCreate(&ctx->events[slot]));
-  HIPCHK(ctx, hipEventRecord(ctx->events[slot], ctx->stream));
+  HIPCHK(ctx, hipEventRecord(ctx->events[slot], stream_of(ctx)));
   return ATM_OK;
 }
 
@@ -1435,7 +1440,7 @@ int atm_event_elapsed_ms(atm_ctx* ctx, int a, int b, float* ms) {
 
 int atm_axpby(atm_ctx* ctx, double* y, const double* x, double a, double b, int64_t n) {
   if (!ctx || !y || !x) return ATM_ERR_ARG;
-  hipLaunchKernelGGL(axpby_kernel, dim3(ew_blocks(n)), dim3(EW_THREADS), 0, ctx->stream, y, x, a, b, n);
+  hipLaunchKernelGGL(axpby_kernel, dim3(ew_blocks(n)), dim3(EW_THREADS), 0, stream_of(ctx), y, x, a, b, n);
   HIPCHK(ctx, hipGetLastError());
   return ATM_OK;
 }
@@ -1463,7 +1468,7 @@ int atm_weightednorm(atm_dg* dg, const double* q, double p, int componentwise, d
   if (rc) return rc;
   const int Ns = dg->Ns;
   const bool inf = std::isinf(p);
-  hipStream_t s = ctx->stream;
+  hipStream_t s = stream_of(ctx);
   HIPCHK(ctx, launch_wnorm(s, q, dg->dev.MJ, Ns, dg->Ntot, p, dg->red));
   HIPCHK(ctx, launch_finalize(s, dg->red, red_blocks(dg->Ntot), Ns, inf ? FIN_MAX : FIN_OUT, 0.0, red_state(dg),
                               red_out(dg)));
@@ -1513,7 +1518,7 @@ int atm_rlsrk_run(atm_dg* dg, int scheme, double* q, double* work, double dt, do
   double bfull[14];
   bfull[t.ns - 1] = t.b[t.ns - 1];
   for (int i = t.ns - 2; i >= 0; --i) bfull[i] = t.a[i + 1] * bfull[i + 1] + t.b[i];
-  hipStream_t s = ctx->stream;
+  hipStream_t s = stream_of(ctx);
   double* st = red_state(dg);
   const int nblk = red_blocks(dg->Ntot);
   HIPCHK(ctx, launch_set_slot(s, st, RX_GAMMA, *gamma));
@@ -1587,7 +1592,7 @@ int atm_batch_create(atm_ctx* ctx, int32_t n, atm_dg* const* members, atm_batch*
 int atm_batch_destroy(atm_batch* b) {
   if (!b) return ATM_ERR_ARG;
   hipSetDevice(b->ctx->device);
-  hipStreamSynchronize(b->ctx->stream);
+  hipStreamSynchronize(stream_of(b->ctx));
   for (hipGraphExec_t& x : b->exec)
     if (x) hipGraphExecDestroy(x);
   for (auto& G : b->groups)
@@ -1625,7 +1630,7 @@ int atm_batch_lsrk_run(atm_batch* b, int scheme, double* const* q, double* const
       start += (dg->dev.K + G.epw - 1) / G.epw;
     }
     G.blocks = start;
-    HIPCHK(ctx, hipMemcpyAsync(G.dtab, tab.data(), tab.size() * sizeof(BatchEntry), hipMemcpyHostToDevice, ctx->stream));
+    HIPCHK(ctx, hipMemcpyAsync(G.dtab, tab.data(), tab.size() * sizeof(BatchEntry), hipMemcpyHostToDevice, stream_of(ctx)));
   }
   const LSRKTable t = lsrk_table(scheme);
   int parity = 0;
@@ -1633,7 +1638,7 @@ int atm_batch_lsrk_run(atm_batch* b, int scheme, double* const* q, double* const
     for (int st = 0; st < t.ns; ++st) {
       for (auto& G : b->groups) {
         if (G.blocks == 0) continue;
-        hipError_t e = G.bfn(G.dtab, (int)G.members.size(), G.blocks, G.threads, G.lds, G.epw, ctx->stream, parity,
+        hipError_t e = G.bfn(G.dtab, (int)G.members.size(), G.blocks, G.threads, G.lds, G.epw, stream_of(ctx), parity,
                              t.a[st], t.a[st] == 0.0 ? 1 : 0, t.b[st]);
         if (e != hipSuccess) return hipfail(ctx, e, "batched element kernel launch");
       }
@@ -1646,7 +1651,7 @@ int atm_batch_lsrk_run(atm_batch* b, int scheme, double* const* q, double* const
   if (use_graph && nsteps >= pair) {
     // the captured kernels read the member tables, so one instantiated graph per
     // scheme serves every later call on this batch
-    hipStream_t s = ctx->stream;
+    hipStream_t s = stream_of(ctx);
     hipGraphExec_t& exec = b->exec[scheme == ATM_LSRK54 ? 0 : 1];
     if (!exec) {
       hipGraph_t graph = nullptr;
@@ -1677,7 +1682,7 @@ int atm_batch_lsrk_run(atm_batch* b, int scheme, double* const* q, double* const
   if (parity != 0) {
     for (int i = 0; i < n; ++i) {
       const int64_t m = (int64_t)b->dgs[i]->Ns * b->dgs[i]->Ntot;
-      hipLaunchKernelGGL(copy_kernel, dim3(ew_blocks(m)), dim3(EW_THREADS), 0, ctx->stream, q[i], scratch[i], m);
+      hipLaunchKernelGGL(copy_kernel, dim3(ew_blocks(m)), dim3(EW_THREADS), 0, stream_of(ctx), q[i], scratch[i], m);
     }
     HIPCHK(ctx, hipGetLastError());
   }
@@ -1741,7 +1746,7 @@ int atm_banded_create(atm_dg* lin, int32_t KV, const int32_t* colelem, int32_t n
     return cleanup(hipfail(ctx, e, "hipMalloc probe"));
   }
   const BandGeo g = band_geo(b);
-  hipStream_t s = ctx->stream;
+  hipStream_t s = stream_of(ctx);
   hipLaunchKernelGGL(fill_kernel, dim3(ew_blocks(bandsz)), dim3(EW_THREADS), 0, s, band, 0.0, (int64_t)bandsz);
   int rc = ATM_OK;
   for (int color = 0; color < b->nb && rc == ATM_OK; ++color) {
@@ -1929,25 +1934,25 @@ int atm_banded_solve(atm_banded* b, double* x, const double* rhs) {
   BandSolveBlkFn bfn = b->blocks ? band_blk_kernel(b->lin->Ns, b->lin->Nq) : nullptr;
   if (mfn && band_mfma_lds_bytes(b->n, b->KV) <= ctx->max_lds) {
     if (b->nwaves16 > 0)
-      hipLaunchKernelGGL(mfn, dim3(b->nwaves16), dim3(64), band_mfma_lds_bytes(b->n, b->KV), ctx->stream, g,
+      hipLaunchKernelGGL(mfn, dim3(b->nwaves16), dim3(64), band_mfma_lds_bytes(b->n, b->KV), stream_of(ctx), g,
                          b->frags, b->perm16, b->wave_mat16, x, rhs);
   } else if (bfn) {
     if (b->nwaves16 > 0)
-      hipLaunchKernelGGL(bfn, dim3(b->nwaves16), dim3(64), 0, ctx->stream, g, b->blocks, b->perm16, b->wave_mat16, x,
+      hipLaunchKernelGGL(bfn, dim3(b->nwaves16), dim3(64), 0, stream_of(ctx), g, b->blocks, b->perm16, b->wave_mat16, x,
                          rhs);
   } else if (BandSolveRegFn fn = band_reg_kernel(b->lin->Ns, b->lin->Nq)) {
     if (b->nwaves > 0)
-      hipLaunchKernelGGL(fn, dim3(b->nwaves), dim3(64), 0, ctx->stream, g, b->fac, b->perm, b->wave_mat, x, rhs);
+      hipLaunchKernelGGL(fn, dim3(b->nwaves), dim3(64), 0, stream_of(ctx), g, b->fac, b->perm, b->wave_mat, x, rhs);
   } else if (lds <= ctx->max_lds) {
     constexpr int LINES = 64 / BAND_SOLVE_PARTS;
     const int64_t blocks = (b->nlines + LINES - 1) / LINES;
     if (blocks > 0)
-      hipLaunchKernelGGL(band_solve_lds_kernel<BAND_SOLVE_PARTS>, dim3(blocks), dim3(64), lds, ctx->stream, g,
+      hipLaunchKernelGGL(band_solve_lds_kernel<BAND_SOLVE_PARTS>, dim3(blocks), dim3(64), lds, stream_of(ctx), g,
                          b->fac, b->line_mat, b->nu, x, rhs);
   } else {
     const int64_t blocks = (b->nlines + 63) / 64;
     if (blocks > 0)
-      hipLaunchKernelGGL(band_solve_kernel, dim3(blocks), dim3(64), 0, ctx->stream, g, b->fac, b->line_mat, b->nu, x,
+      hipLaunchKernelGGL(band_solve_kernel, dim3(blocks), dim3(64), 0, stream_of(ctx), g, b->fac, b->line_mat, b->nu, x,
                          rhs);
   }
   HIPCHK(ctx, hipGetLastError());
@@ -1970,7 +1975,7 @@ int atm_banded_download(atm_banded* b, double* host) {
   atm_ctx* ctx = b->lin->ctx;
   const size_t sz = (size_t)b->nu * b->n * b->nb;
   std::vector<double> tmp(sz);
-  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  HIPCHK(ctx, hipStreamSynchronize(stream_of(ctx)));
   HIPCHK(ctx, hipMemcpy(tmp.data(), b->fac, sz * sizeof(double), hipMemcpyDeviceToHost));
   for (int64_t line = 0; line < b->nlines; ++line) {
     const int64_t u = b->line_mat_host[line];
@@ -2001,7 +2006,7 @@ int atm_ark23_run(atm_dg* ex, atm_dg* lin, atm_banded* band, double* q, double* 
   double* Qs[3] = {q, work + n, work + 2 * n};
   double* exK[3] = {work + 3 * n, work + 4 * n, work + 5 * n};
   double* imK[3] = {work + 6 * n, work + 7 * n, work + 8 * n};
-  hipStream_t s = ctx->stream;
+  hipStream_t s = stream_of(ctx);
   const int blocks = ew_blocks(n);
   for (int64_t step = 0; step < nsteps; ++step) {
     for (int i = 0; i < 3; ++i) {

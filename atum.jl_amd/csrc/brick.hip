@@ -191,7 +191,7 @@ int atm_dg_create_brick(atm_ctx* ctx, const atm_dg_desc* d, const atm_brick_desc
   }
   for (int i = 0; i < 3; ++i) b.coords[i] = cdev[i];
   if (Ntot > 0) {
-    hipStream_t s = ctx->stream;
+    hipStream_t s = stream_of(ctx);
     hipLaunchKernelGGL(brick_volume_kernel, dim3(grid_blocks(Ntot)), dim3(256), 0, s, b, (double*)pts, (double*)g,
                        (double*)MJ, (double*)MJI);
     hipLaunchKernelGGL(brick_face_kernel, dim3(grid_blocks(NF)), dim3(256), 0, s, b, (int64_t*)fxp, (double*)fMJ,
@@ -235,7 +235,7 @@ int atm_dg_geometry(atm_dg* dg, int which, void* host) {
   if (bytes == 0) return ATM_OK;
   if (!src) return fail(dg->ctx, ATM_ERR_UNSUPPORTED, "atm_dg_geometry: array not held on the device");
   hipSetDevice(dg->ctx->device);
-  HIPCHK(dg->ctx, hipStreamSynchronize(dg->ctx->stream));
+  HIPCHK(dg->ctx, hipStreamSynchronize(stream_of(dg->ctx)));
   HIPCHK(dg->ctx, hipMemcpy(host, src, bytes, hipMemcpyDeviceToHost));
   return ATM_OK;
 }

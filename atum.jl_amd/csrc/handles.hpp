@@ -48,6 +48,13 @@ struct atm_dg {
 
 namespace atm {
 
+// Stream override of the calling thread: while set, every launch, copy and
+// wait of the library on this thread uses it instead of the context's stream
+// (the serving scheduler runs its requests on its own streams this way,
+// leaving the context's stream to the caller's thread).
+extern thread_local hipStream_t tl_stream;
+inline hipStream_t stream_of(const atm_ctx* c) { return tl_stream ? tl_stream : c->stream; }
+
 inline int fail(atm_ctx* ctx, int code, const std::string& msg) {
   if (ctx) ctx->err = msg;
   return code;
@@ -68,6 +75,9 @@ inline int hipfail(atm_ctx* ctx, hipError_t e, const char* what) {
 // and creates the operator with its device description filled in (geometry,
 // aux and boundary-state pointers still null).
 int dg_prepare(atm_ctx* ctx, const atm_dg_desc* d, atm_dg** out);
+
+// The linear operator a banded factorisation was built from.
+atm_dg* banded_lin(const atm_banded* b);
 
 // Device allocation owned by the operator (freed by atm_dg_destroy).
 int dg_alloc(atm_dg* dg, size_t bytes, void** out);

@@ -42,18 +42,18 @@ int atm_snapshot(atm_ctx* ctx, int slot, void* host_dst, void* staging, const vo
   int rc = ensure_copy_stream(ctx, slot);
   if (rc) return rc;
   // the slot's previous copy must have left the staging buffer before it is overwritten
-  HIPCHK(ctx, hipStreamWaitEvent(ctx->stream, ctx->snap_done[slot], 0));
+  HIPCHK(ctx, hipStreamWaitEvent(stream_of(ctx), ctx->snap_done[slot], 0));
   const void* from = src;
   if (staging) {
-    HIPCHK(ctx, hipMemcpyAsync(staging, src, bytes, hipMemcpyDeviceToDevice, ctx->stream));
+    HIPCHK(ctx, hipMemcpyAsync(staging, src, bytes, hipMemcpyDeviceToDevice, stream_of(ctx)));
     from = staging;
   }
-  HIPCHK(ctx, hipEventRecord(ctx->snap_ready[slot], ctx->stream));
+  HIPCHK(ctx, hipEventRecord(ctx->snap_ready[slot], stream_of(ctx)));
   HIPCHK(ctx, hipStreamWaitEvent(ctx->copy, ctx->snap_ready[slot], 0));
   HIPCHK(ctx, hipMemcpyAsync(host_dst, from, bytes, hipMemcpyDeviceToHost, ctx->copy));
   HIPCHK(ctx, hipEventRecord(ctx->snap_done[slot], ctx->copy));
   // without staging, later work on the compute stream may not modify src before the copy
-  if (!staging) HIPCHK(ctx, hipStreamWaitEvent(ctx->stream, ctx->snap_done[slot], 0));
+  if (!staging) HIPCHK(ctx, hipStreamWaitEvent(stream_of(ctx), ctx->snap_done[slot], 0));
   return ATM_OK;
 }
 

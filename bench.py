@@ -253,7 +253,7 @@ def serve_workload(args):
                        "kinds": kinds, "chunk_steps": args.chunk, "imex_streams": args.imex_streams},
             "p50_s": s["p50_s"], "p99_s": s["p99_s"], "mean_s": s["mean_s"], "completed": s["completed"],
             "wall_s": s["wall_s"], "ticks": s["ticks"], "max_active": s["max_active"],
-            "breakdown_s": {k: s[k] for k in ("admit_s", "enqueue_s", "wait_s", "retire_s", "gpu_span_s")}}
+            "breakdown_s": {k: s[k] for k in ("build_s", "admit_s", "enqueue_s", "wait_s", "retire_s", "gpu_span_s")}}
 
 
 def copy_workload(args):

@@ -222,6 +222,91 @@ int atm_batch_ngroups(const atm_batch* b);
 int atm_batch_lsrk_run(atm_batch* b, int scheme, double* const* q, double* const* dq,
                        double* const* scratch, const double* dt, int64_t nsteps, int use_graph);
 
+/* ---- serving scheduler (continuous batching on one device) ------------------
+ * Simulation requests are submitted from any host thread.  A worker thread of
+ * the server admits them in submission order (while fewer than max_active are
+ * resident and their device footprint fits hbm_budget), advances every resident
+ * request by a common chunk of steps per tick (the LSRK requests in one batched
+ * launch per kernel group and stage as atm_batch_lsrk_run, the ARK23 requests on
+ * side streams beside it), and retires the finished ones: the final state goes
+ * to the request's host result buffer and its device memory is freed.  A
+ * request's result is bitwise that of its steps run alone with atm_lsrk_run /
+ * atm_ark23_run.  The server launches on streams of its own; the context's
+ * stream stays with the caller.  Work that a request's operators still need
+ * (creation, factorisation) must be enqueued on the context's stream of the
+ * submitting thread, or finished, before atm_server_submit.  The operators must
+ * outlive their requests.  The worker captures no graphs (HIP would refuse other
+ * threads' copies meanwhile), and no graph may be captured on the context while
+ * the server holds requests.  No counterpart in the reference (it runs one solver
+ * per process): these entry points serve many independent runs at once. */
+typedef struct atm_server atm_server;
+typedef struct {
+  int32_t max_active;   /* requests resident at once, >= 1 */
+  int32_t chunk;        /* steps per tick, >= 1 */
+  int32_t imex_streams; /* side streams for ARK23 requests, 0..8 (0: the main stream) */
+  int32_t reserved;     /* 0 */
+  double hbm_budget;    /* bytes of device memory the resident requests may hold */
+} atm_server_desc;
+
+#define ATM_REQ_LSRK 0  /* atm_lsrk_run(dg, scheme, ...) */
+#define ATM_REQ_ARK23 1 /* atm_ark23_run(dg, lin, band, ...) */
+
+typedef struct {
+  int32_t kind;         /* ATM_REQ_LSRK or ATM_REQ_ARK23 */
+  int32_t scheme;       /* LSRK: ATM_LSRK54 or ATM_LSRK144 */
+  int32_t split_rhs;    /* ARK23, as atm_ark23_run */
+  int32_t paperversion; /* ARK23, as atm_ark23_run */
+  atm_dg* dg;           /* LSRK: the operator; ARK23: the explicit operator or NULL */
+  atm_dg* lin;          /* ARK23: the linear operator */
+  atm_banded* band;     /* ARK23: the factors of lin for this dt */
+  const double* q0;     /* host initial state, Ns*nnodes doubles (copied by submit) */
+  double* result;       /* host buffer for the final state (NULL: discarded); written
+                           before the request reports ATM_REQ_DONE */
+  double dt;
+  int64_t steps;        /* >= 0 */
+} atm_request;
+
+/* request states (negative: the request failed with that status) */
+#define ATM_REQ_QUEUED 0
+#define ATM_REQ_ACTIVE 1
+#define ATM_REQ_DONE 2
+
+typedef struct {
+  int64_t submitted, completed, failed, ticks, max_active;
+  double dof_updates; /* sum over steps taken of Ns*nnodes*stages (LSRK54 5, LSRK144 14, ARK23 3) */
+  double busy_s;      /* wall time the worker spent in ticks */
+  double gpu_span_s;  /* device time from the first to the last launch of each tick */
+  double admit_s;     /* worker: allocating and uploading admitted requests */
+  double enqueue_s;   /* worker: launching the ticks */
+  double wait_s;      /* worker: waiting for the device */
+  double retire_s;    /* worker: downloading results, freeing */
+} atm_server_stats;
+
+int atm_server_create(atm_ctx* ctx, const atm_server_desc* desc, atm_server** out);
+/* finishes every submitted request, then stops the worker */
+int atm_server_destroy(atm_server* s);
+/* copies q0; validates the request (operators of this context, footprint within
+ * the budget); *id identifies it from then on */
+int atm_server_submit(atm_server* s, const atm_request* req, int64_t* id);
+/* the request's state (ATM_REQ_*) or its negative error status */
+int atm_server_poll(atm_server* s, int64_t id);
+/* blocks until the request is done or failed; returns as atm_server_poll */
+int atm_server_wait(atm_server* s, int64_t id);
+/* the next request that finished or failed and was not yet handed out here:
+ * returns 1 and sets *id, or 0 when timeout_s (< 0: no limit) passes first */
+int atm_server_next_done(atm_server* s, double timeout_s, int64_t* id);
+/* blocks until every submitted request is done or failed */
+int atm_server_drain(atm_server* s);
+/* seconds on the server's clock (0 at creation) */
+double atm_server_clock(atm_server* s);
+/* submission, admission and completion times of a request (server clock; -1: not yet) */
+int atm_server_times(atm_server* s, int64_t id, double* times3);
+/* message of a failed request ("" otherwise) */
+const char* atm_server_error(atm_server* s, int64_t id);
+/* forgets a finished or failed request */
+int atm_server_release(atm_server* s, int64_t id);
+int atm_server_get_stats(atm_server* s, atm_server_stats* out);
+
 /* ---- timing: HIP events recorded on the context's stream ------------------
  * slot in [0, 16); atm_event_elapsed_ms waits for the later event. */
 int atm_event_record(atm_ctx* ctx, int slot);
@@ -233,7 +318,7 @@ int atm_axpby(atm_ctx* ctx, double* y, const double* x, double a, double b, int6
 
 /* ---- brick meshes built on the device ---------------------------------------
  * Replaces brickgrid(LobattoCell, coordinates; periodic, ordering) (Bennu,
- * called at test/euler/isentropicvortex.jl:43 and experiment/*) followed by
+ * called at test/euler/isentropicvortex.jl:43 and the experiment/ scripts) followed by
  * DGSEM(...) (src/dgsem.jl:38-56): the library builds the node coordinates,
  * metric terms, MJ, MJI, faceMJ, normals, face tables and boundary tags on the
  * device, bit-identical to the host builder.  Vertex coordinates, LGL nodes and
