@@ -392,9 +392,18 @@ struct BatchEntry {
   int64_t block_start;
 };
 
-ATM_DEV int batch_find(const BatchEntry* __restrict__ tab, int n) {
+// Workgroups are dealt to the 8 XCDs round-robin (workgroup w runs on XCD w % 8),
+// each XCD with its own L2.  The logical block index gives XCD x a contiguous
+// range of blocks (consecutive elements, whose faces neighbour each other), so a
+// neighbour's face states are read from the same L2.  A bijection of [0, G).
+ATM_DEV int64_t xcd_block() {
+  const int64_t G = gridDim.x, w = blockIdx.x;
+  const int64_t x = w & 7, i = w >> 3, base = G >> 3, rem = G & 7;
+  return x * base + (x < rem ? x : rem) + i;
+}
+
+ATM_DEV int batch_find(const BatchEntry* __restrict__ tab, int n, int64_t b) {
   int lo = 0, hi = n - 1;
-  const int64_t b = blockIdx.x;
   while (lo < hi) {
     const int mid = (lo + hi + 1) >> 1;
     if (tab[mid].block_start <= b) lo = mid;
@@ -409,16 +418,17 @@ ATM_DEV int batch_find(const BatchEntry* __restrict__ tab, int n) {
       NAME(const DevDG p, const double* __restrict__ q, double* __restrict__ dq, double alpha, int assign, \
            double* __restrict__ qout, double bdt, int epw) {                                              \
     extern __shared__ double lds[];                                                                       \
-    BODY<TARGS>(p, lds, blockIdx.x, q, dq, alpha, assign, qout, bdt, epw);                                \
+    BODY<TARGS>(p, lds, xcd_block(), q, dq, alpha, assign, qout, bdt, epw);                               \
   }                                                                                                       \
   template <TPARAMS>                                                                                      \
   __global__ void __launch_bounds__(512)                                                                  \
       NAME##_batch(const BatchEntry* __restrict__ tab, int n, int parity, double alpha, int assign,      \
                    double b, int epw) {                                                                   \
     extern __shared__ double lds[];                                                                       \
-    const int i = batch_find(tab, n);                                                                     \
+    const int64_t lb = xcd_block();                                                                       \
+    const int i = batch_find(tab, n, lb);                                                                 \
     const DevDG p = tab[i].p;                                                                             \
-    BODY<TARGS>(p, lds, (int64_t)blockIdx.x - tab[i].block_start, tab[i].buf[parity], tab[i].dq, alpha,   \
+    BODY<TARGS>(p, lds, lb - tab[i].block_start, tab[i].buf[parity], tab[i].dq, alpha,                    \
                 assign, tab[i].buf[1 - parity], b * tab[i].dt, epw);                                      \
   }
 
