@@ -57,6 +57,17 @@ def read_snapshot(path):
     return a.reshape(head["shape"]), head
 
 
+def load_checkpoint(path, dg):
+    """Resume from a snapshot: (device field of `dg`, step, time).  The solvers of
+    this package carry no state between steps beyond q and the time (LSRK's first
+    stage overwrites dq, ARK23 and RLSRK steps start from q alone), so a run
+    continued from a snapshot with `t0 = time` repeats the uninterrupted run bit
+    for bit."""
+    a, head = read_snapshot(path)
+    q = dg.fieldarray(a)
+    return q, int(head.get("step", 0)), float(head.get("time", 0.0))
+
+
 class PinnedBuffer:
     """Page-locked host memory from the library (hipHostMalloc), viewed as numpy."""
 

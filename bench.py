@@ -36,6 +36,19 @@ def algorithmic_bytes_per_stage(dim, Nq, K, Ns, naux):
     return per_elem * K
 
 
+FP64_PEAK_TFS = 78.6  # MI355X FP64 (vector and matrix)
+
+
+def ec_volume_flops_per_stage(dim, Nq, K, Ns, gravity):
+    """FP64 operations of the EC flux-differencing volume term as the reference evaluates
+    it (+, -, *, /, sqrt, log = 1 each): per node, direction and partner node (the
+    node itself included) one two-point flux, 3d^2 + 16d + 48 (+15 with gravity), and
+    its contraction with the metric weights, Ns (6d + 2)."""
+    pair = 3 * dim * dim + 16 * dim + 48 + (15 if gravity else 0)
+    contraction = Ns * (6 * dim + 2)
+    return K * Nq ** dim * dim * Nq * (pair + contraction)
+
+
 def bubble_setup(Kd, N, precision="fast"):
     import numpy as np  # noqa: F401
     import atum_hip as A
@@ -378,6 +391,13 @@ def main():
                      "kernel": "fused stage kernel (volume + surface + LSRK update)",
                      "algorithmic_bytes_per_launch": algo, "avg_launch_us": kern_s * 1e6},
     }
+    flops = ec_volume_flops_per_stage(3, N + 1, K, Ns, True)
+    out["roofline_fp64"] = {"bound": "fp64", "achieved": flops / kern_s / 1e12, "peak": FP64_PEAK_TFS,
+                            "unit": "TFLOP/s", "frac": flops / kern_s / 1e12 / FP64_PEAK_TFS,
+                            "flops_per_launch": flops,
+                            "note": "EC volume operations in the reference's evaluation count (each "
+                                    "ordered pair, diagonal included); the fused kernel evaluates each "
+                                    "unordered pair once and also does the surface term and the update"}
     if args.precision == "fast" and not args.no_reference_order:
         _, _, _, ms_ref = timed("reference")
         out["reference_order"] = {"value": dof * stages / (ms_ref / 1e3), "unit": "DOF-updates/s",

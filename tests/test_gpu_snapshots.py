@@ -58,3 +58,31 @@ def test_snapshot_without_staging_and_query():
     dg.ctx.check(_lib.lib.atm_snapshot_wait(dg.ctx.h, 5), "wait")
     assert _lib.lib.atm_snapshot_query(dg.ctx.h, 5) == 1
     np.testing.assert_array_equal(host.array, q0)
+
+
+def test_checkpoint_restart_lsrk_and_ark(tmp_path):
+    """Checkpoint/resume: a run stopped at a snapshot and continued from it (new
+    field, new solver with t0 = the snapshot's time) ends bit for bit where the
+    uninterrupted run ends, for LSRK54 and for IMEX ARK23 with banded solves."""
+    import atum_hip as A
+    from serve_requests import _imex
+    dg, q0, dt = _vortex()
+    ex, lin, qi0, dti = _imex(2, 3, 3, 7)({})
+    cases = [("lsrk", dg, q0, dt, lambda d, q, t0: A.LSRK54(d, q, dt, t0=t0)),
+             ("ark", lin, qi0, dti, lambda d, q, t0: A.ARK23(ex, lin, q, dti, t0=t0, split_rhs=False))]
+    for name, d, qa, h, make in cases:
+        full = d.fieldarray(qa)
+        make(d, full, 0.0).run(full, 12)
+        q = d.fieldarray(qa)
+        solver = make(d, q, 0.0)
+        w = A.SnapshotWriter(q, str(tmp_path / (name + "_{step:04d}.atm")), every=7, solver=solver)
+        for step in range(1, 8):
+            t = solver.time
+            solver.dostep(q)
+            w(step, t, q)
+        (path,) = w.close()
+        q2, step, t0 = A.load_checkpoint(path, d)
+        assert step == 7 and t0 == solver.time
+        make(d, q2, t0).run(q2, 5)
+        d.ctx.sync()
+        np.testing.assert_array_equal(q2.numpy(), full.numpy(), err_msg=name)
