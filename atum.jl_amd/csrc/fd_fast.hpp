@@ -25,28 +25,27 @@
 
 namespace atm {
 
-// f/atanh(f) = sum_k LMG[k] u^k and atanh(f)/f = sum_k u^k/(2k+1), u = f^2
+// Series coefficients in u = f^2: LM_COEF[k] of f/atanh(f) = sum_k LM_COEF[k] u^k and
+// LM_COEF[8 + k] = 1/(2k+1) of atanh(f)/f = sum_k u^k/(2k+1).  Kept in constant memory:
+// the kernel loads them once into scalar registers, where literal 64-bit constants
+// would be rebuilt in vector registers at every use inside the pair loop.
+static __constant__ double LM_COEF[16] = {1.0, -0.33333333333333331, -0.088888888888888892, -0.046560846560846560,
+                                   -0.030194003527336862, -0.021796804019026242, -0.016787551856334924,
+                                   -0.013502765051265932, 1.0, 1.0 / 3, 1.0 / 5, 1.0 / 7, 1.0 / 9, 1.0 / 11,
+                                   1.0 / 13, 1.0 / 15};
 __device__ __forceinline__ double lm_inv_series(double u) {
 #pragma clang fp contract(fast)
-  const double c[8] = {1.0,
-                       -0.33333333333333331,
-                       -0.088888888888888892,
-                       -0.046560846560846560,
-                       -0.030194003527336862,
-                       -0.021796804019026242,
-                       -0.016787551856334924,
-                       -0.013502765051265932};
-  double s = c[7];
+  double s = LM_COEF[7];
 #pragma unroll
-  for (int k = 6; k >= 0; --k) s = s * u + c[k];
+  for (int k = 6; k >= 0; --k) s = s * u + LM_COEF[k];
   return s;
 }
 
 __device__ __forceinline__ double lm_series(double u) {
 #pragma clang fp contract(fast)
-  double s = 1.0 / 15;
+  double s = LM_COEF[15];
 #pragma unroll
-  for (int k = 6; k >= 0; --k) s = s * u + 1.0 / (2 * k + 1);
+  for (int k = 6; k >= 0; --k) s = s * u + LM_COEF[8 + k];
   return s;
 }
 
