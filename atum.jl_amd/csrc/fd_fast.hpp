@@ -172,7 +172,7 @@ ATM_DEV void fd_fast_body(const DevDG& p, double* lds, int64_t blk, const double
   constexpr int Np = D == 1 ? NQ : (D == 2 ? NQ * NQ : NQ * NQ * NQ);
   constexpr int M = Nq / 2;  // pair slots per node and direction
   double* sD = lds;
-  constexpr int stride = (NP + D + M * Ns) * Np;
+  constexpr int stride = (NP + D) * Np + fast_xbuf<L, NQ>();
   const int tid = threadIdx.x;
   const int el = tid / Np;
   const int ijk = tid - el * Np;
@@ -212,7 +212,10 @@ ATM_DEV void fd_fast_body(const DevDG& p, double* lds, int64_t blk, const double
 #pragma unroll
   for (int r = 0; r < D; ++r) {
     if (!((p.dirmask >> r) & 1)) continue;
-    __syncthreads();
+    // No barrier before overwriting slg: every thread passed the previous
+    // direction's post-pair barrier, so all reads of the old slg are done; the
+    // slots are rewritten only after the next barrier, which every thread
+    // reaches after gathering its previous slots.
     if (active) {
 #pragma unroll
       for (int d = 0; d < D; ++d) slg[d * Np + ijk] = MJn * gall[r][d];
