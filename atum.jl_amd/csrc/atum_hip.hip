@@ -1136,7 +1136,13 @@ int atm_ctx_set_stream(atm_ctx* ctx, void* hip_stream) {
 
 void* atm_ctx_get_stream(atm_ctx* ctx) { return ctx ? (void*)ctx->stream : nullptr; }
 
-const char* atm_last_error(atm_ctx* ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+const char* atm_last_error(atm_ctx* ctx) {
+  // the calling thread's copy stays valid until its next call here
+  thread_local std::string copy;
+  if (!ctx) return "null context";
+  copy = last_error(ctx);
+  return copy.c_str();
+}
 
 int atm_sync(atm_ctx* ctx) {
   if (!ctx) return ATM_ERR_ARG;

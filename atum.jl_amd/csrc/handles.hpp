@@ -3,6 +3,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -16,7 +17,8 @@ struct atm_ctx {
   hipStream_t own = nullptr;
   hipStream_t stream = nullptr;
   size_t max_lds = 65536;
-  std::string err;
+  std::string err;      // last error message (any thread; guarded by err_mu)
+  std::mutex err_mu;
   hipEvent_t events[16] = {};
   // snapshots: a second stream (default priority) for device-to-host copies
   hipStream_t copy = nullptr;
@@ -56,7 +58,10 @@ extern thread_local hipStream_t tl_stream;
 inline hipStream_t stream_of(const atm_ctx* c) { return tl_stream ? tl_stream : c->stream; }
 
 inline int fail(atm_ctx* ctx, int code, const std::string& msg) {
-  if (ctx) ctx->err = msg;
+  if (ctx) {
+    std::lock_guard<std::mutex> lk(ctx->err_mu);
+    ctx->err = msg;
+  }
   return code;
 }
 
@@ -78,6 +83,12 @@ int dg_prepare(atm_ctx* ctx, const atm_dg_desc* d, atm_dg** out);
 
 // The linear operator a banded factorisation was built from.
 atm_dg* banded_lin(const atm_banded* b);
+
+// A copy of the context's last error message (threads may set it concurrently).
+inline std::string last_error(atm_ctx* ctx) {
+  std::lock_guard<std::mutex> lk(ctx->err_mu);
+  return ctx->err;
+}
 
 // Device allocation owned by the operator (freed by atm_dg_destroy).
 int dg_alloc(atm_dg* dg, size_t bytes, void** out);

@@ -93,7 +93,7 @@ void free_buffers(Req* r) {
 
 void mark_failed(atm_server* s, Req* r, int rc, const char* what) {
   r->frc = rc < 0 ? rc : ATM_ERR_HIP;
-  r->fmsg = std::string(what) + ": " + s->ctx->err;
+  r->fmsg = std::string(what) + ": " + last_error(s->ctx);
 }
 
 // Device side of an admitted request, on the server's main stream.

@@ -137,6 +137,8 @@ int atm_stream_sync(atm_ctx* ctx, void* stream);
 /* Work enqueued on `waiter` after this call starts after the work enqueued on
  * `waitee` before it (an event, no host wait); NULL names the context's stream. */
 int atm_stream_wait(atm_ctx* ctx, void* waiter, void* waitee);
+/* the context's last error message; the string stays valid until the calling
+ * thread's next atm_last_error */
 const char* atm_last_error(atm_ctx* ctx);
 int atm_sync(atm_ctx* ctx);
 const char* atm_version(void);

@@ -105,3 +105,25 @@ def test_context_side_stream_tendency():
     ctx.sync()
     ctx.destroy_stream(s)
     np.testing.assert_array_equal(a.numpy(), b.numpy())
+
+
+def test_server_refuses_invalid_requests():
+    """Requests the scheduler cannot serve are refused at submit, before anything is
+    queued: a footprint above the HBM budget, an operator of another context."""
+    import atum_hip as A
+    r = make_stream(duration=0.05, rate=200.0, seed=3, mix=(1.0, 0.0, 0.0), steps=(1, 1), vortex_k=(2, 2),
+                    nrange=(2, 2))[0]
+    small = A.Server(hbm_budget=1024.0)
+    with pytest.raises(A.AtumError, match="budget"):
+        small.submit(r)
+    assert small.stats()["submitted"] == 0
+    small.close()
+    other = A.Server(ctx=A.Context(0))
+    with pytest.raises(A.AtumError, match="another context"):
+        other.submit(r)  # the request's operator lives on the default context
+    other.close()
+    ok = A.Server()
+    ok.submit(r)
+    done = ok.next_done(30.0)
+    assert done is r and r.error is None
+    ok.close()
