@@ -88,6 +88,11 @@ _sig("atm_dg_geometry", c_int, [c_void_p, c_int, c_void_p])
 _sig("atm_dg_destroy", c_int, [c_void_p])
 _sig("atm_dg_nstates", c_int, [c_void_p])
 _sig("atm_dg_nnodes", c_int64, [c_void_p])
+_sig("atm_dg_get_epw", c_int, [c_void_p])
+_sig("atm_dg_set_epw", c_int, [c_void_p, c_int])
+_sig("atm_dg_epw_candidates", c_int, [c_void_p, ctypes.POINTER(ctypes.c_int32), c_int])
+_sig("atm_dg_autotune", c_int, [c_void_p, c_void_p, c_void_p, c_int, ctypes.POINTER(ctypes.c_int32),
+                                ctypes.POINTER(ctypes.c_float), c_int])
 _sig("atm_dg_tendency", c_int, [c_void_p, c_void_p, c_void_p, c_double, c_int])
 _sig("atm_lsrk_stage", c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_double, c_double, c_double])
 _sig("atm_lsrk_run", c_int, [c_void_p, c_int, c_void_p, c_void_p, c_void_p, c_double, c_double,

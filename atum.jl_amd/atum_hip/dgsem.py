@@ -97,6 +97,9 @@ class DGSEM:
         d.naux = naux
         d.precision = _lib.PRECISION_IDS[precision]
         self.precision = precision
+        # what selects the element kernel (with K, the key of the workgroup-shape tuning)
+        self.kernel_key = (law.law, dim, grid.N, d.volume_form, d.volume_flux, d.surface_flux, d.dirmask,
+                           precision)
         d.gamma = float(getattr(law, "gamma", 1.4))
         d.grav = float(getattr(law, "grav", 9.81))
         d.omega = float(prob.omega)
@@ -173,6 +176,21 @@ class DGSEM:
     def lsrk_stage(self, q_in: Field, q_out: Field, dq: Field, a: float, bdt: float, time=0.0):
         self.ctx.check(_lib.lib.atm_lsrk_stage(self.h, q_in.ptr, q_out.ptr, dq.ptr, float(a),
                                                float(bdt), float(time)), "atm_lsrk_stage")
+
+    @property
+    def epw(self):
+        """Elements per workgroup of the element kernel (results do not depend on it)."""
+        return int(_lib.lib.atm_dg_get_epw(self.h))
+
+    @epw.setter
+    def epw(self, value):
+        self.ctx.check(_lib.lib.atm_dg_set_epw(self.h, int(value)), "atm_dg_set_epw")
+
+    def epw_candidates(self):
+        """The workgroup shapes that fit (whole waves, <= 512 threads, the device's LDS)."""
+        out = (ctypes.c_int32 * 8)()
+        n = _lib.lib.atm_dg_epw_candidates(self.h, out, 8)
+        return [int(out[i]) for i in range(n)]
 
     def fieldarray(self, values=None):
         f = Field(self.Ns, self.grid.Ntot, self.ctx)

@@ -24,6 +24,7 @@ import numpy as np
 
 from . import _lib
 from .odesolvers import ARK23, LSRK54, BandedOperator
+from .tuning import TuneCache, autotune
 
 c_int32, c_int64, c_double, c_void_p = ctypes.c_int32, ctypes.c_int64, ctypes.c_double, ctypes.c_void_p
 
@@ -113,8 +114,13 @@ class ServeStats:
 
 class Server:
     def __init__(self, ctx=None, hbm_budget=0.9 * 288e9, max_active=512, chunk=10, keep_results=True,
-                 imex_streams=3):
+                 imex_streams=3, tune: TuneCache | None = None):
+        """tune: a TuneCache: every new operator takes its elements per workgroup from
+        it, timed once per kernel configuration and size class on first use (while the
+        device serves other requests, so the timing is taken under load)."""
         self.ctx = ctx or _lib.context()
+        self.tune = tune
+        self._tuned = set()
         self.keep_results = keep_results
         self.imex_streams = int(imex_streams)
         self.cache = {}
@@ -145,6 +151,11 @@ class Server:
             req.kind, req.scheme, req.dg = REQ_LSRK, 0, dg.h
             shape_dg = dg
         q0 = np.ascontiguousarray(q0, dtype=np.float64)
+        if self.tune is not None:
+            for op in ((built[0], built[1]) if r.kind == "imex" else (built[0],)):
+                if op is not None and id(op) not in self._tuned:
+                    autotune(op, q0, reps=10, cache=self.tune)
+                    self._tuned.add(id(op))
         ns, nn = int(_lib.lib.atm_dg_nstates(shape_dg.h)), int(_lib.lib.atm_dg_nnodes(shape_dg.h))
         if q0.size != ns * nn:
             raise ValueError(f"request {r.rid}: initial state has {q0.size} values, the operator {ns * nn}")

@@ -1,0 +1,73 @@
+"""Workgroup-shape autotuner for the element kernels.
+
+The number of elements per workgroup (epw) of an operator's element kernel
+changes how the device is filled, never the results (each element is computed
+by its own threads in a fixed order).  `autotune` times the shapes that fit
+(`atm_dg_autotune`: tendency launches timed with HIP events) and keeps the
+fastest; a `TuneCache` remembers the choice per kernel configuration and mesh
+size class, in memory or in a JSON file, so later operators of the same kind
+skip the timing.
+"""
+from __future__ import annotations
+
+import ctypes
+import json
+import math
+import os
+
+from . import _lib
+from .fields import Field
+
+
+def _size_class(K: int) -> int:
+    """Element counts within a factor of two share a tuning."""
+    return 0 if K <= 0 else int(round(math.log2(K)))
+
+
+class TuneCache:
+    """Chosen epw per (kernel configuration, size class); persisted to `path` if given."""
+
+    def __init__(self, path: str | None = None):
+        self.path = path
+        self.table = {}
+        if path and os.path.exists(path):
+            with open(path) as f:
+                self.table = json.load(f)
+
+    @staticmethod
+    def key(dg) -> str:
+        return json.dumps(list(dg.kernel_key) + [_size_class(dg.grid.K)])
+
+    def get(self, dg):
+        return self.table.get(self.key(dg))
+
+    def put(self, dg, epw: int):
+        self.table[self.key(dg)] = int(epw)
+        if self.path:
+            tmp = self.path + ".tmp"
+            with open(tmp, "w") as f:
+                json.dump(self.table, f, indent=1, sort_keys=True)
+            os.replace(tmp, self.path)
+
+
+def autotune(dg, q, reps: int = 20, cache: TuneCache | None = None):
+    """Choose the operator's elements per workgroup by timing; returns (epw, {epw: us per launch}).
+    `q` is a representative state (Field or host array of the operator's shape); the
+    timings are empty when the cache already holds the choice."""
+    if cache is not None:
+        hit = cache.get(dg)
+        if hit is not None and hit in dg.epw_candidates():
+            dg.epw = hit
+            return hit, {}
+    qf = q if isinstance(q, Field) else dg.fieldarray(q)
+    dq = dg.fieldarray()
+    cap = 8
+    ep = (ctypes.c_int32 * cap)()
+    us = (ctypes.c_float * cap)()
+    best = _lib.lib.atm_dg_autotune(dg.h, qf.ptr, dq.ptr, int(reps), ep, us, cap)
+    dg.ctx.check(best if best < 0 else 0, "atm_dg_autotune")
+    n = len(dg.epw_candidates())
+    times = {int(ep[i]): float(us[i]) for i in range(min(n, cap))}
+    if cache is not None:
+        cache.put(dg, best)
+    return int(best), times

@@ -1239,8 +1239,12 @@ int dg_prepare(atm_ctx* ctx, const atm_dg_desc* d, atm_dg** out) {
   dg->bfn = r.bfn;
   dg->lds = r.lds;
   dg->epw = r.epw;
+  dg->lds_fixed = r.lds_fixed;
+  dg->lds_elem = r.lds_elem;
   dg->mfn = resolve_measures(d->law, d->dim);
   DevDG& p = dg->dev;
+  p.epw = r.epw;
+  p.lds = (int)r.lds;
   p.c.gamma = d->gamma;
   p.c.grav = d->grav;
   p.c.omega = d->omega;
@@ -1343,6 +1347,91 @@ int atm_dg_destroy(atm_dg* dg) {
 }
 
 int atm_dg_nstates(const atm_dg* dg) { return dg ? dg->Ns : ATM_ERR_ARG; }
+
+int atm_dg_get_epw(const atm_dg* dg) { return dg ? dg->epw : ATM_ERR_ARG; }
+
+int atm_dg_set_epw(atm_dg* dg, int epw) {
+  if (!dg) return ATM_ERR_ARG;
+  const size_t lds = dg->lds_fixed + (size_t)(epw > 0 ? epw : 0) * dg->lds_elem;
+  if (epw < 1 || epw > 512 || epw * dg->Np > 512 || lds > dg->ctx->max_lds)
+    return fail(dg->ctx, ATM_ERR_ARG, "atm_dg_set_epw: the workgroup would exceed 512 threads or the LDS");
+  if (epw == dg->epw) return ATM_OK;
+  hipSetDevice(dg->ctx->device);
+  HIPCHK(dg->ctx, hipStreamSynchronize(stream_of(dg->ctx)));
+  if (dg->lsrk_exec) hipGraphExecDestroy(dg->lsrk_exec);  // captured with the old launch shape
+  dg->lsrk_exec = nullptr;
+  dg->lsrk_scheme = -1;
+  dg->epw = epw;
+  dg->lds = lds;
+  dg->dev.epw = epw;
+  dg->dev.lds = (int)lds;
+  return ATM_OK;
+}
+
+int atm_dg_epw_candidates(const atm_dg* dg, int32_t* out, int max) {
+  if (!dg || (max > 0 && !out)) return ATM_ERR_ARG;
+  // whole waves of 64 lanes: one element per 1, 2, 4 or 8 waves for large elements,
+  // 64/128/256/512 lanes of small elements otherwise
+  int cand[8], n = 0;
+  auto add = [&](int e) {
+    if (e < 1 || e * dg->Np > 512 || dg->lds_fixed + (size_t)e * dg->lds_elem > dg->ctx->max_lds) return;
+    for (int i = 0; i < n; ++i)
+      if (cand[i] == e) return;
+    cand[n++] = e;
+  };
+  if (dg->Np >= 64) {
+    for (int e : {1, 2, 4}) add(e);
+  } else {
+    for (int t : {64, 128, 256, 512}) add(t / dg->Np);
+  }
+  for (int i = 0; i < n && i < max; ++i) out[i] = cand[i];
+  return n;
+}
+
+int atm_dg_autotune(atm_dg* dg, const double* q, double* dq, int reps, int32_t* epw_out, float* us_out, int max) {
+  if (!dg || !q || !dq || reps < 1) return fail(dg ? dg->ctx : nullptr, ATM_ERR_ARG, "atm_dg_autotune: bad argument");
+  atm_ctx* ctx = dg->ctx;
+  int32_t cand[8];
+  const int n = atm_dg_epw_candidates(dg, cand, 8);
+  if (n < 1) return fail(ctx, ATM_ERR_UNSUPPORTED, "atm_dg_autotune: no workgroup shape fits");
+  hipSetDevice(ctx->device);
+  hipEvent_t a = nullptr, b = nullptr;
+  HIPCHK(ctx, hipEventCreate(&a));
+  hipError_t e = hipEventCreate(&b);
+  if (e != hipSuccess) {
+    hipEventDestroy(a);
+    return hipfail(ctx, e, "hipEventCreate");
+  }
+  const int first = dg->epw;
+  int best = first, rc = ATM_OK;
+  float best_ms = 0.f;
+  const hipStream_t s = stream_of(ctx);
+  for (int i = 0; i < n && rc == ATM_OK; ++i) {
+    if ((rc = atm_dg_set_epw(dg, cand[i]))) break;
+    for (int w = 0; w < 2 && rc == ATM_OK; ++w) rc = launch(dg, q, dq, 1.0, 1, nullptr, 0.0);
+    if (rc) break;
+    hipEventRecord(a, s);
+    for (int k = 0; k < reps && rc == ATM_OK; ++k) rc = launch(dg, q, dq, 1.0, 1, nullptr, 0.0);
+    hipEventRecord(b, s);
+    if (rc) break;
+    float ms = 0.f;
+    if ((e = hipEventSynchronize(b)) != hipSuccess || (e = hipEventElapsedTime(&ms, a, b)) != hipSuccess) {
+      rc = hipfail(ctx, e, "atm_dg_autotune timing");
+      break;
+    }
+    if (epw_out && i < max) epw_out[i] = cand[i];
+    if (us_out && i < max) us_out[i] = ms * 1e3f / reps;
+    if (i == 0 || ms < best_ms) {
+      best_ms = ms;
+      best = cand[i];
+    }
+  }
+  hipEventDestroy(a);
+  hipEventDestroy(b);
+  const int rc2 = atm_dg_set_epw(dg, rc == ATM_OK ? best : first);
+  if (rc) return rc;
+  return rc2 ? rc2 : best;
+}
 int64_t atm_dg_nnodes(const atm_dg* dg) { return dg ? dg->Ntot : ATM_ERR_ARG; }
 
 int atm_dg_tendency(atm_dg* dg, double* dq, const double* q, double time, int increment) {

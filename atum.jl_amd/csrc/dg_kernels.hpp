@@ -22,6 +22,7 @@ struct DevDG {
   LawConst c;
   int K, Nq, Np, Nfpf, Nfp, naux, dirmask, vflux, sflux;
   int64_t Ntot, NF;
+  int epw, lds;     // elements per workgroup and LDS bytes of a single-operator launch
   const double* D;  // Nq*Nq column major
   const double* g;
   const double* MJ;
@@ -464,11 +465,10 @@ inline int elements_per_workgroup(int Np) { return Np >= 256 ? 1 : 256 / Np; }
 template <class L, int NQ, int VF, int SF>
 hipError_t launch_fd(const DevDG& p, hipStream_t st, const double* q, double* dq, double alpha,
                      int assign, double* qout, double bdt) {
-  constexpr int D = L::D, NP = PairFlux<L, VF>::NP;
-  const int epw = elements_per_workgroup(p.Np);
+  const int epw = p.epw;
   const int threads = ((epw * p.Np + 63) / 64) * 64;
-  if (threads > 512) return hipErrorInvalidConfiguration;
-  const size_t lds = sizeof(double) * ((size_t)p.Nq * p.Nq + (size_t)epw * (NP + D) * p.Np);
+  if (epw < 1 || threads > 512) return hipErrorInvalidConfiguration;
+  const size_t lds = p.lds;
   const int blocks = (p.K + epw - 1) / epw;
   if (blocks == 0) return hipSuccess;
   hipLaunchKernelGGL((fd_kernel<L, NQ, VF, SF>), dim3(blocks), dim3(threads), lds, st, p, q, dq,
@@ -479,11 +479,10 @@ hipError_t launch_fd(const DevDG& p, hipStream_t st, const double* q, double* dq
 template <class L, int NQ, int SF>
 hipError_t launch_weak(const DevDG& p, hipStream_t st, const double* q, double* dq, double alpha,
                        int assign, double* qout, double bdt) {
-  constexpr int D = L::D, Ns = L::Ns;
-  const int epw = elements_per_workgroup(p.Np);
+  const int epw = p.epw;
   const int threads = ((epw * p.Np + 63) / 64) * 64;
-  if (threads > 512) return hipErrorInvalidConfiguration;
-  const size_t lds = sizeof(double) * ((size_t)p.Nq * p.Nq + (size_t)epw * D * Ns * p.Np);
+  if (epw < 1 || threads > 512) return hipErrorInvalidConfiguration;
+  const size_t lds = p.lds;
   const int blocks = (p.K + epw - 1) / epw;
   if (blocks == 0) return hipSuccess;
   hipLaunchKernelGGL((weak_kernel<L, NQ, SF>), dim3(blocks), dim3(threads), lds, st, p, q, dq,
@@ -491,41 +490,49 @@ hipError_t launch_weak(const DevDG& p, hipStream_t st, const double* q, double* 
   return hipGetLastError();
 }
 
-// LDS bytes a configuration needs (checked against the device limit at create time)
+// LDS bytes per element a configuration needs (the workgroup adds Nq*Nq doubles
+// for D); the total is checked against the device limit whenever epw is chosen
 template <class L>
-size_t lds_bytes(int form, int vflux, int Nq, int Np) {
+size_t lds_elem_bytes(int form, int vflux, int Np) {
   constexpr int D = L::D, Ns = L::Ns;
-  const int epw = elements_per_workgroup(Np);
-  if (form == ATM_FORM_WEAK) return sizeof(double) * ((size_t)Nq * Nq + (size_t)epw * D * Ns * Np);
+  if (form == ATM_FORM_WEAK) return sizeof(double) * (size_t)D * Ns * Np;
   int np = D * Ns;  // central: physical flux per node
   if (vflux != ATM_FLUX_CENTRAL) np = (L::LAW == ATM_LAW_SHALLOW_WATER) ? 2 * D + 3 : D + 4;
-  return sizeof(double) * ((size_t)Nq * Nq + (size_t)epw * (np + D) * Np);
+  return sizeof(double) * (size_t)(np + D) * Np;
 }
 
+// A kernel family for an operator: launchers, the default elements per
+// workgroup and the LDS bytes of a workgroup, lds_fixed + epw * lds_elem.
 struct Resolved {
   LaunchFn fn;
-  size_t lds;
+  size_t lds;  // at the default epw
   int ns;
   BatchLaunchFn bfn;
-  int epw;  // elements per workgroup of the launch
+  int epw;     // default elements per workgroup
+  size_t lds_fixed, lds_elem;
 };
+
+inline Resolved make_resolved(LaunchFn fn, BatchLaunchFn bfn, int ns, int epw, size_t fixed, size_t elem) {
+  return Resolved{fn, fixed + (size_t)epw * elem, ns, bfn, epw, fixed, elem};
+}
 
 template <class L, int NQ, int VF, int SF>
 Resolved make_fd(int Nq, int Np) {
-  return Resolved{launch_fd<L, NQ, VF, SF>, lds_bytes<L>(ATM_FORM_FLUX_DIFFERENCING, VF, Nq, Np), L::Ns,
-                  launch_fd_batch<L, NQ, VF, SF>, elements_per_workgroup(Np)};
+  return make_resolved(launch_fd<L, NQ, VF, SF>, launch_fd_batch<L, NQ, VF, SF>, L::Ns, elements_per_workgroup(Np),
+                       sizeof(double) * Nq * Nq, lds_elem_bytes<L>(ATM_FORM_FLUX_DIFFERENCING, VF, Np));
 }
 
 template <class L, int NQ, int SF>
 Resolved make_weak(int Nq, int Np) {
-  return Resolved{launch_weak<L, NQ, SF>, lds_bytes<L>(ATM_FORM_WEAK, 0, Nq, Np), L::Ns,
-                  launch_weak_batch<L, NQ, SF>, elements_per_workgroup(Np)};
+  return make_resolved(launch_weak<L, NQ, SF>, launch_weak_batch<L, NQ, SF>, L::Ns, elements_per_workgroup(Np),
+                       sizeof(double) * Nq * Nq, lds_elem_bytes<L>(ATM_FORM_WEAK, 0, Np));
 }
 
 // Generic (runtime Nq, runtime surface flux) kernel for a law/dimension.
 template <class L>
 Resolved resolve_generic(int form, int vflux, int Nq, int Np) {
-  Resolved r{nullptr, lds_bytes<L>(form, vflux, Nq, Np), L::Ns, nullptr, elements_per_workgroup(Np)};
+  Resolved r = make_resolved(nullptr, nullptr, L::Ns, elements_per_workgroup(Np), sizeof(double) * Nq * Nq,
+                             lds_elem_bytes<L>(form, vflux, Np));
   if (form == ATM_FORM_WEAK) {
     r.fn = launch_weak<L, 0, -1>;
     r.bfn = launch_weak_batch<L, 0, -1>;

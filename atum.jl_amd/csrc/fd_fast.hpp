@@ -292,11 +292,13 @@ constexpr int fast_epw() {
   return Np >= 64 ? 1 : 256 / Np;
 }
 
+// LDS bytes per element (primitives, metric weights, exchange slots); the
+// workgroup adds NQ*NQ doubles for D
 template <class L, int NQ>
-size_t fast_lds_bytes() {
+constexpr size_t fast_lds_elem_bytes() {
   constexpr int D = L::D, NP = FastEC<L::D, true>::NP;
   constexpr int Np = D == 1 ? NQ : (D == 2 ? NQ * NQ : NQ * NQ * NQ);
-  return sizeof(double) * ((size_t)NQ * NQ + (size_t)fast_epw<L, NQ>() * ((NP + D) * Np + fast_xbuf<L, NQ>()));
+  return sizeof(double) * ((size_t)(NP + D) * Np + fast_xbuf<L, NQ>());
 }
 
 template <class L, int NQ, int SF>
@@ -304,12 +306,12 @@ hipError_t launch_fd_fast(const DevDG& p, hipStream_t st, const double* q, doubl
                           int assign, double* qout, double bdt) {
   constexpr int D = L::D;
   constexpr int Np = D == 1 ? NQ : (D == 2 ? NQ * NQ : NQ * NQ * NQ);
-  constexpr int epw = fast_epw<L, NQ>();
-  constexpr int threads = ((epw * Np + 63) / 64) * 64;
-  static_assert(threads <= 512, "workgroup exceeds the kernel's launch bound");
+  const int epw = p.epw;
+  const int threads = ((epw * Np + 63) / 64) * 64;
+  if (epw < 1 || threads > 512) return hipErrorInvalidConfiguration;
   const int blocks = (p.K + epw - 1) / epw;
   if (blocks == 0) return hipSuccess;
-  const size_t lds = fast_lds_bytes<L, NQ>();
+  const size_t lds = p.lds;
   hipLaunchKernelGGL((fd_fast_kernel<L, NQ, SF>), dim3(blocks), dim3(threads), lds, st, p, q, dq, alpha,
                      assign, qout, bdt, epw);
   return hipGetLastError();

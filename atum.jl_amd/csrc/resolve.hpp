@@ -19,7 +19,8 @@ Resolved resolve_fast_euler_gravity(int dim, int sflux, int Nq);
 
 template <class L, int NQ>
 Resolved fast_entry(int sflux) {
-  Resolved r{nullptr, fast_lds_bytes<L, NQ>(), L::Ns, nullptr, fast_epw<L, NQ>()};
+  Resolved r = make_resolved(nullptr, nullptr, L::Ns, fast_epw<L, NQ>(), sizeof(double) * NQ * NQ,
+                             fast_lds_elem_bytes<L, NQ>());
   if (sflux == ATM_FLUX_ROE) {
     r.fn = launch_fd_fast<L, NQ, ATM_FLUX_ROE>;
     r.bfn = launch_fd_fast_batch<L, NQ, ATM_FLUX_ROE>;

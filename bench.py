@@ -253,7 +253,8 @@ def serve_workload(args):
     from serve_requests import make_stream
 
     reqs = make_stream(duration=args.serve_seconds)
-    server = A.Server(chunk=args.chunk, keep_results=False, imex_streams=args.imex_streams)
+    server = A.Server(chunk=args.chunk, keep_results=False, imex_streams=args.imex_streams,
+                      tune=A.TuneCache() if args.serve_tune else None)
     stats = server.run(reqs)
     s = stats.summary()
     kinds = {k: sum(r.kind == k for r in reqs) for k in ("vortex", "sw", "imex")}
@@ -263,7 +264,8 @@ def serve_workload(args):
             "vs_baseline": None, "dtype": "f64", "data": "synthetic",
             "config": {"workload": f"Poisson 50/s for {args.serve_seconds} s, mix 50/30/20 "
                                    "vortex/shallow-water/IMEX, steps U{50..500}", "requests": len(reqs),
-                       "kinds": kinds, "chunk_steps": args.chunk, "imex_streams": args.imex_streams},
+                       "kinds": kinds, "chunk_steps": args.chunk, "imex_streams": args.imex_streams,
+                       "autotune": bool(args.serve_tune)},
             "p50_s": s["p50_s"], "p99_s": s["p99_s"], "mean_s": s["mean_s"], "completed": s["completed"],
             "wall_s": s["wall_s"], "ticks": s["ticks"], "max_active": s["max_active"],
             "breakdown_s": {k: s[k] for k in ("build_s", "admit_s", "enqueue_s", "wait_s", "retire_s", "gpu_span_s")}}
@@ -316,6 +318,11 @@ def main():
     ap.add_argument("--instances", type=int, default=256)
     ap.add_argument("--serve-seconds", type=float, default=10.0)
     ap.add_argument("--chunk", type=int, default=10, help="steps per serving tick")
+    ap.add_argument("--no-tune", dest="tune", action="store_false",
+                    help="bubble: keep the default elements per workgroup (no autotuning before timing)")
+    ap.add_argument("--serve-tune", action="store_true",
+                    help="serve: tune each new operator's workgroup shape on first use (under load; "
+                         "measured slower on the default stream)")
     ap.add_argument("--imex-streams", type=int, default=3,
                     help="side streams for the IMEX requests of the served stream (0: the main stream)")
     args = ap.parse_args()
@@ -337,9 +344,14 @@ def main():
 
     Kd, N = args.elements, args.N
 
+    tuning = {}
+
     def timed(precision):
         grid, dg, q0, dt = bubble_setup(Kd, N, precision)
         ctx = dg.ctx
+        if args.tune:  # elements per workgroup chosen by timing, before the timed region
+            best, times = A.autotune(dg, q0, reps=50)
+            tuning[precision] = {"epw": best, "us_per_launch": times}
         q = dg.fieldarray(q0)
         solver = A.LSRK54(dg, q, dt, use_graph=not args.no_graph)
         solver.run(q, args.warmup)
@@ -385,7 +397,8 @@ def main():
         "config": {"workload": f"3-D Euler-gravity rising bubble, {Kd}^3 hex elements, N={N}, "
                                "EC flux differencing + Roe surface, reflective walls, LSRK54",
                    "elements": K, "dof": dof, "stages_per_step": 5, "dt": dt,
-                   "hipgraph": not args.no_graph, "precision": args.precision},
+                   "hipgraph": not args.no_graph, "precision": args.precision,
+                   "elements_per_workgroup": dg.epw, "autotune": tuning.get(args.precision)},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "kernel": "fused stage kernel (volume + surface + LSRK update)",
@@ -403,6 +416,7 @@ def main():
         out["reference_order"] = {"value": dof * stages / (ms_ref / 1e3), "unit": "DOF-updates/s",
                                   "ms_per_step": ms_ref / args.steps,
                                   "avg_launch_us": ms_ref / 1e3 / stages * 1e6,
+                                  "autotune": tuning.get("reference"),
                                   "note": "same run with the reference's summation order and no FMA"}
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(Kd, N, args.cpu_stages)

@@ -156,6 +156,21 @@ int atm_dg_create(atm_ctx* ctx, const atm_dg_desc* desc, atm_dg** out);
 int atm_dg_destroy(atm_dg* dg);
 /* number of state components Ns of the law (numberofstates, src/balancelaw.jl:14) */
 int atm_dg_nstates(const atm_dg* dg);
+/* ---- workgroup shape and autotuning ----------------------------------------
+ * Elements per workgroup (epw) of the operator's element kernel: the default
+ * fills 256 lanes with small elements and gives large ones a workgroup each.
+ * atm_dg_set_epw changes it (results do not depend on it: every element is
+ * computed by its own threads in the same order).  atm_dg_epw_candidates lists
+ * the shapes that fit (whole waves, at most 512 threads, the LDS); it returns
+ * their count.  atm_dg_autotune times `reps` tendency launches dq = L(q) per
+ * candidate with HIP events on the context's stream, keeps the fastest and
+ * returns it (per-candidate epw and microseconds per launch go to the optional
+ * arrays of capacity max); dq is overwritten.  No counterpart in the reference
+ * (KernelAbstractions workgroup sizes are fixed in src/dgsem.jl:74-76). */
+int atm_dg_get_epw(const atm_dg* dg);
+int atm_dg_set_epw(atm_dg* dg, int epw);
+int atm_dg_epw_candidates(const atm_dg* dg, int32_t* out, int max);
+int atm_dg_autotune(atm_dg* dg, const double* q, double* dq, int reps, int32_t* epw_out, float* us_out, int max);
 /* total nodes Np*K */
 int64_t atm_dg_nnodes(const atm_dg* dg);
 

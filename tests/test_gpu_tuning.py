@@ -1,0 +1,102 @@
+"""Workgroup-shape autotuning (atm_dg_set_epw / atm_dg_autotune): every shape
+gives bit-identical tendencies and time steps, the tuner keeps one of the
+shapes it timed, and a batch of members with different shapes still reproduces
+each member alone bit for bit."""
+import json
+
+import numpy as np
+import pytest
+
+import problems as P
+from gpu_cases import Case
+
+pytestmark = pytest.mark.gpu
+
+CONFIGS = [("euler", 2, 3, 4, "ec", "rusanov", "reference"), ("euler", 2, 4, 3, "ec", "rusanov", "fast"),
+           ("shallow_water", 2, 5, 3, "weak", "rusanov", "reference"), ("advection", 1, 3, 6, "central", "rusanov",
+                                                                       "reference"),
+           ("euler_gravity", 3, 4, 2, "ec", "roe", "fast"), ("euler", 3, 2, 3, "weak", "roe", "reference")]
+
+
+def _dg(law, dim, N, K, form, sflux, precision):
+    import atum_hip as A
+    c = Case(law, dim, N, K, form=form, sflux=sflux, seed=N + K, periodic=(True,) * dim)
+    if precision == "reference":
+        return c.dg, c.q
+    dg = A.DGSEM(law=c.dg.law, grid=c.grid, volume_form=c.dg.volume_form,
+                 surface_numericalflux=c.dg.surface_numericalflux, auxstate=c.aux, precision="fast")
+    return dg, c.q
+
+
+@pytest.mark.parametrize("cfg", CONFIGS)
+def test_every_shape_gives_identical_results(cfg):
+    import atum_hip as A
+    dg, q0 = _dg(*cfg)
+    default = dg.epw
+    cands = dg.epw_candidates()
+    assert default in cands and len(cands) >= 2
+    q = dg.fieldarray(q0)
+    ref_dq, ref_q = None, None
+    for e in cands + [default]:
+        dg.epw = e
+        assert dg.epw == e
+        dq = dg.fieldarray()
+        dg(dq, q, 0.0, increment=False)
+        qs = dg.fieldarray(q0)
+        A.LSRK54(dg, qs, 1e-4).run(qs, 4)  # the step graph is re-captured for the new shape
+        dg.ctx.sync()
+        if ref_dq is None:
+            ref_dq, ref_q = dq.numpy(), qs.numpy()
+        np.testing.assert_array_equal(dq.numpy(), ref_dq, err_msg=f"epw={e}")
+        np.testing.assert_array_equal(qs.numpy(), ref_q, err_msg=f"epw={e}")
+
+
+def test_invalid_shape_refused():
+    import atum_hip as A
+    dg, _ = _dg("euler_gravity", 3, 4, 2, "ec", "roe", "fast")
+    before = dg.epw
+    for bad in (0, -1, 5, 1000):  # 5 * 125 threads > 512
+        with pytest.raises(A.AtumError):
+            dg.epw = bad
+    assert dg.epw == before
+
+
+def test_autotune_and_cache(tmp_path):
+    import atum_hip as A
+    dg, q0 = _dg("euler", 2, 4, 8, "ec", "rusanov", "fast")
+    path = str(tmp_path / "tune.json")
+    cache = A.TuneCache(path)
+    best, times = A.autotune(dg, q0, reps=5, cache=cache)
+    assert best in dg.epw_candidates() and dg.epw == best
+    assert sorted(times) == sorted(dg.epw_candidates()) and all(t > 0 for t in times.values())
+    assert best == min(times, key=times.get)
+    stored = json.load(open(path))
+    assert list(stored.values()) == [best]
+    # a second operator of the same kind takes the stored choice without timing
+    dg2, _ = _dg("euler", 2, 4, 8, "ec", "rusanov", "fast")
+    hit, t2 = A.autotune(dg2, q0, cache=A.TuneCache(path))
+    assert hit == best and t2 == {} and dg2.epw == best
+
+
+def test_batch_members_with_different_shapes():
+    import atum_hip as A
+    dgs, qs0 = [], []
+    for k, e in ((3, None), (4, 0), (5, -1)):
+        dg, q0 = _dg("euler", 2, 3, k, "ec", "rusanov", "fast")
+        if e is not None:
+            dg.epw = dg.epw_candidates()[e]
+        dgs.append(dg)
+        qs0.append(q0)
+    assert len({d.epw for d in dgs}) >= 2
+    b = A.Batch(dgs)
+    assert b.ngroups >= 2  # one group per shape
+    qs = [d.fieldarray(q) for d, q in zip(dgs, qs0)]
+    dqs = [d.fieldarray() for d in dgs]
+    scr = [d.fieldarray() for d in dgs]
+    dts = [1e-3, 2e-3, 3e-3]
+    b.lsrk_run(qs, dqs, scr, dts, 6)
+    for d, q0, q, dt in zip(dgs, qs0, qs, dts):
+        alone = d.fieldarray(q0)
+        A.LSRK54(d, alone, dt).run(alone, 6)
+        d.ctx.sync()
+        np.testing.assert_array_equal(q.numpy(), alone.numpy())
