@@ -238,6 +238,13 @@ ATM_DEV void fd_body(const DevDG& p, double* lds, int64_t blk, const double* __r
     for (int k = 0; k < NP; ++k) sP[k * Np + ijk] = P1[k];
     MJn = p.MJ[node];
   }
+  // metric rows of every direction in the mask, loaded with the state (one latency)
+  double gall[D][D];
+#pragma unroll
+  for (int r = 0; r < D; ++r)
+#pragma unroll
+    for (int d = 0; d < D; ++d)
+      gall[r][d] = (active && ((p.dirmask >> r) & 1)) ? p.g[(int64_t)(r + D * d) * p.Ntot + node] : 0.0;
   int idx[3];
   node_index<D>(ijk, Nq, idx);
   const double MJIn = active ? p.MJI[node] : 0.0;
@@ -249,7 +256,7 @@ ATM_DEV void fd_body(const DevDG& p, double* lds, int64_t blk, const double* __r
     __syncthreads();  // primitives written / previous direction's metric row consumed
     if (active) {
 #pragma unroll
-      for (int d = 0; d < D; ++d) slg[d * Np + ijk] = MJn * p.g[(int64_t)(r + D * d) * p.Ntot + node];
+      for (int d = 0; d < D; ++d) slg[d * Np + ijk] = MJn * gall[r][d];
     }
     __syncthreads();
     if (!active) continue;
@@ -325,6 +332,13 @@ ATM_DEV void weak_body(const DevDG& p, double* lds, int64_t blk, const double* _
 #pragma unroll
     for (int s = 0; s < Ns; ++s) q1[s] = q[s * p.Ntot + node];
     load_aux<L>(p, node, a1);
+    // metric rows of the directions in the mask, loaded once (not per state)
+    double gl[D][D];
+#pragma unroll
+    for (int r = 0; r < D; ++r)
+#pragma unroll
+      for (int dd = 0; dd < D; ++dd)
+        gl[r][dd] = ((p.dirmask >> r) & 1) ? p.g[(int64_t)(r + D * dd) * p.Ntot + node] : 0.0;
     double F[D][Ns];
     L::flux(p.c, q1, a1, F);
     const double MJn = p.MJ[node];
@@ -336,7 +350,7 @@ ATM_DEV void weak_body(const DevDG& p, double* lds, int64_t blk, const double* _
         if (!((p.dirmask >> r) & 1)) continue;
         double v = 0;
 #pragma unroll
-        for (int dd = 0; dd < D; ++dd) v += p.g[(int64_t)(r + D * dd) * p.Ntot + node] * F[dd][s];
+        for (int dd = 0; dd < D; ++dd) v += gl[r][dd] * F[dd][s];
         v *= MJn;
         slF[(slot * Ns + s) * Np + ijk] = v;
         ++slot;

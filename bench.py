@@ -213,6 +213,50 @@ def imex_workload(args):
                        "dof": dof, "dt": dt, "lines": 32 * 32 * 16, "precision": args.precision}}
 
 
+def weak_workload(args):
+    """One large weak-form instance (the HBM-bound kernel family): 2-D shallow water,
+    --weak-K^2 elements on the periodic unit square, N = --N, weak volume + Rusanov,
+    LSRK54 fused stages, elements per workgroup tuned before timing."""
+    import numpy as np
+    import atum_hip as A
+    import problems as P
+
+    K1, N = args.weak_K, args.N
+    v = P.julia_range(0.0, 1.0, K1 + 1)
+    grid = A.brickgrid(A.LobattoCell(N, 2), (v, v), (True, True), lazy=True)
+    dg = A.DGSEM(law=A.ShallowWaterLaw(2, grav=9.81), grid=grid, surface_numericalflux=A.RusanovFlux(),
+                 geometry="device")
+    q0 = P.sw_bump(grid.points, 0.05, 0.1, (0.5, 0.5))
+    tuned = A.autotune(dg, q0, reps=20) if args.tune else None
+    dt = P.sw_ensemble_dt(K1, N, 0.05)
+    q = dg.fieldarray(q0)
+    s_ = A.LSRK54(dg, q, dt)
+    s_.run(q, 4)
+    dg.ctx.sync()
+    dg.ctx.record(0)
+    s_.run(q, args.steps)
+    dg.ctx.record(1)
+    ms = dg.ctx.elapsed_ms(0, 1)
+    if not np.all(np.isfinite(q.numpy())):
+        raise SystemExit("bench: non-finite shallow-water state")
+    dof = q0.size
+    stages = 5 * args.steps
+    algo = algorithmic_bytes_per_stage(2, N + 1, grid.K, 4, 0)
+    kern_s = ms / 1e3 / stages
+    achieved = algo / kern_s / 1e9
+    return {"metric": "FP64 DOF-updates/s per RK stage (weak form, one instance)",
+            "value": dof * stages / (ms / 1e3), "unit": "DOF-updates/s", "n_gpus": 1, "steps": args.steps,
+            "warmup": 4, "ms_per_step": ms / args.steps, "higher_is_better": True, "scaling": "weak",
+            "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+            "config": {"workload": f"2-D shallow water {K1}^2 elements, N={N}, weak form + Rusanov, LSRK54",
+                       "dof": dof, "elements_per_workgroup": dg.epw,
+                       "autotune_us_per_launch": tuned[1] if tuned else None},
+            "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                         "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                         "kernel": "fused weak stage kernel (volume + surface + LSRK update)",
+                         "algorithmic_bytes_per_launch": algo, "avg_launch_us": kern_s * 1e6}}
+
+
 def vortex_workload(args):
     """2-D isentropic vortex, [-5,5]^2 periodic, 8x8, N = 4, EC + Rusanov, LSRK54, CFL 0.1."""
     import numpy as np
@@ -313,9 +357,11 @@ def main():
                          "reference: the reference's summation order, no FMA")
     ap.add_argument("--no-reference-order", action="store_true",
                     help="skip the extra timing of the reference-order kernel")
-    ap.add_argument("--workload", choices=["bubble", "ensemble", "imex", "vortex", "serve", "copy"], default="bubble",
+    ap.add_argument("--workload", choices=["bubble", "ensemble", "imex", "vortex", "serve", "copy", "weak"],
+                    default="bubble",
                     help="bubble (default, the metric's configuration); the others are extra lines")
     ap.add_argument("--instances", type=int, default=256)
+    ap.add_argument("--weak-K", type=int, default=256, help="weak workload: elements per direction")
     ap.add_argument("--serve-seconds", type=float, default=10.0)
     ap.add_argument("--chunk", type=int, default=10, help="steps per serving tick")
     ap.add_argument("--no-tune", dest="tune", action="store_false",
@@ -338,7 +384,7 @@ def main():
 
     if args.workload != "bubble":
         fn = {"ensemble": ensemble_workload, "imex": imex_workload, "vortex": vortex_workload,
-              "serve": serve_workload, "copy": copy_workload}[args.workload]
+              "serve": serve_workload, "copy": copy_workload, "weak": weak_workload}[args.workload]
         print(json.dumps(fn(args)))
         return
 
