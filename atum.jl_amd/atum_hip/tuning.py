@@ -71,3 +71,49 @@ def autotune(dg, q, reps: int = 20, cache: TuneCache | None = None):
     if cache is not None:
         cache.put(dg, best)
     return int(best), times
+
+
+def autotune_batch(dgs, q0s, dts, steps: int = 2, rounds: int = 1):
+    """Choose elements per workgroup for batched launches: the members sharing a kernel
+    configuration form one launch group, so each group gets one shape, found by
+    coordinate descent over the groups (each candidate timed on a full batched run of
+    `steps` LSRK54 steps from copies of the initial states, the other groups at their
+    current shapes).  Returns {kernel configuration: (epw, {epw: ms per run})}."""
+    from .batch import Batch, LSRK54
+
+    if not dgs:
+        return {}
+    ctx = dgs[0].ctx
+    groups = {}
+    for d in dgs:
+        groups.setdefault(d.kernel_key, []).append(d)
+    qs = [d.fieldarray(q) for d, q in zip(dgs, q0s)]
+    dqs = [d.fieldarray() for d in dgs]
+    scr = [d.fieldarray() for d in dgs]
+
+    def run_ms():
+        b = Batch(dgs, ctx)
+        for q, d, q0 in zip(qs, dgs, q0s):
+            q.upload(q0)
+        b.lsrk_run(qs, dqs, scr, dts, steps, LSRK54)  # capture and warm up
+        ctx.sync()
+        ctx.record(12)
+        b.lsrk_run(qs, dqs, scr, dts, steps, LSRK54)
+        ctx.record(13)
+        return ctx.elapsed_ms(12, 13)
+
+    out = {}
+    for _ in range(rounds):
+        for key, members in groups.items():
+            times = {}
+            for e in members[0].epw_candidates():
+                if not all(e in m.epw_candidates() for m in members):
+                    continue
+                for m in members:
+                    m.epw = e
+                times[e] = run_ms()
+            best = min(times, key=times.get)
+            for m in members:
+                m.epw = best
+            out[key] = (best, times)
+    return out

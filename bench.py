@@ -118,6 +118,9 @@ def ensemble_workload(args):
     ctx = dgs[0].ctx
     dof = sum(q.size for q in q0s)
     stages = 5 * args.steps
+    tuned = None
+    if args.ensemble_tune:  # one workgroup shape per launch group, chosen on the batched run itself
+        tuned = A.autotune_batch(dgs, q0s, dts)
     qs = [dg.fieldarray(q) for dg, q in zip(dgs, q0s)]
     ens = A.LSRKEnsemble(dgs, qs, dts)
     ens.run(2)
@@ -153,7 +156,8 @@ def ensemble_workload(args):
         "vs_baseline": None, "dtype": "f64", "data": "synthetic",
         "config": {"workload": f"{len(dgs)} shallow-water instances, K~U{{8..64}}^2, N~U{{3..7}}, "
                                "weak form + Rusanov, LSRK54, per-instance dt (PCG64 seed 20260101)",
-                   "dof": dof, "kernel_groups": ens.batch.ngroups},
+                   "dof": dof, "kernel_groups": ens.batch.ngroups,
+                   "elements_per_workgroup": {f"N={k[2]}": v[0] for k, v in tuned.items()} if tuned else None},
         "per_request_eager": {"value": dof * stages / (per[False] / 1e3), "ms_per_step": per[False] / args.steps},
         "per_request_graph": {"value": dof * stages / (per[True] / 1e3), "ms_per_step": per[True] / args.steps},
         "speedup_vs_per_request_eager": per[False] / ms_b,
@@ -193,6 +197,9 @@ def imex_workload(args):
     z = np.zeros_like(rho)
     q0 = np.stack([rho, z, z, z, rho * (cv * T + aux[0])])
     dt = 0.2 * grid.min_node_distance(dims=(0, 1)) / 330.0
+    tuned = {}
+    if args.tune:  # both operators' workgroup shapes, before the factorisation and the timing
+        tuned = {"explicit": A.autotune(ex, q0, reps=20), "linear": A.autotune(lin, q0, reps=20)}
     q = ex.fieldarray(q0)
     ark = A.ARK23(ex, lin, q, dt, split_rhs=False)
     ctx = ex.ctx
@@ -210,7 +217,9 @@ def imex_workload(args):
             "ms_per_step": ms / args.steps, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
             "dtype": "f64", "data": "synthetic",
             "config": {"workload": "IMEX Euler-gravity 32x32x10, N=3, ARK23 unsplit, banded vertical solve",
-                       "dof": dof, "dt": dt, "lines": 32 * 32 * 16, "precision": args.precision}}
+                       "dof": dof, "dt": dt, "lines": 32 * 32 * 16, "precision": args.precision,
+                       "elements_per_workgroup": {"explicit": ex.epw, "linear": lin.epw},
+                       "autotune_us_per_launch": {k: v[1] for k, v in tuned.items()} or None}}
 
 
 def weak_workload(args):
@@ -270,6 +279,8 @@ def vortex_workload(args):
                  surface_numericalflux=A.RusanovFlux(), precision=args.precision)
     q0 = P.isentropic_vortex(grid.points, 0.0, center=(0.0, 0.0), u0=(1.0, 1.0))
     dt = 0.1 * (10 / K) / N
+    if args.tune:
+        A.autotune(dg, q0, reps=50)
     q = dg.fieldarray(q0)
     s_ = A.LSRK54(dg, q, dt)
     s_.run(q, 4)
@@ -286,7 +297,8 @@ def vortex_workload(args):
             "ms_per_step": ms / args.steps, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
             "dtype": "f64", "data": "synthetic",
             "config": {"workload": "2-D isentropic vortex 8x8, N=4, EC + Rusanov, LSRK54 (CFL 0.1)",
-                       "dof": dof, "us_per_stage": ms / (5 * args.steps) * 1e3, "precision": args.precision}}
+                       "dof": dof, "us_per_stage": ms / (5 * args.steps) * 1e3, "precision": args.precision,
+                       "elements_per_workgroup": dg.epw}}
 
 
 def serve_workload(args):
@@ -366,6 +378,9 @@ def main():
     ap.add_argument("--chunk", type=int, default=10, help="steps per serving tick")
     ap.add_argument("--no-tune", dest="tune", action="store_false",
                     help="bubble: keep the default elements per workgroup (no autotuning before timing)")
+    ap.add_argument("--ensemble-tune", action="store_true",
+                    help="ensemble: tune one workgroup shape per launch group on the batched run "
+                         "(measured within noise of the default shapes)")
     ap.add_argument("--serve-tune", action="store_true",
                     help="serve: tune each new operator's workgroup shape on first use (under load; "
                          "measured slower on the default stream)")

@@ -100,3 +100,28 @@ def test_batch_members_with_different_shapes():
         A.LSRK54(d, alone, dt).run(alone, 6)
         d.ctx.sync()
         np.testing.assert_array_equal(q.numpy(), alone.numpy())
+
+
+def test_autotune_batch_groups():
+    """Batch-level tuning: one shape per launch group (members of one kernel
+    configuration share it), results of the tuned batch equal each member alone."""
+    import atum_hip as A
+    dgs, q0s = [], []
+    for law, N, K in (("euler", 3, 3), ("euler", 3, 5), ("shallow_water", 4, 4)):
+        d, q = _dg(law, 2, N, K, "ec" if law == "euler" else "weak", "rusanov",
+                   "fast" if law == "euler" else "reference")
+        dgs.append(d)
+        q0s.append(q)
+    dts = [1e-3, 1e-3, 5e-4]
+    out = A.autotune_batch(dgs, q0s, dts, steps=2)
+    assert len(out) == 2
+    for key, (best, times) in out.items():
+        assert best in times and best == min(times, key=times.get)
+        assert all(d.epw == best for d in dgs if d.kernel_key == key)
+    ens = A.LSRKEnsemble(dgs, [d.fieldarray(q) for d, q in zip(dgs, q0s)], dts)
+    ens.run(4)
+    for d, q0, q, dt in zip(dgs, q0s, ens.qs, dts):
+        alone = d.fieldarray(q0)
+        A.LSRK54(d, alone, dt).run(alone, 4)
+        d.ctx.sync()
+        np.testing.assert_array_equal(q.numpy(), alone.numpy())
