@@ -89,6 +89,7 @@ _sig("atm_dg_destroy", c_int, [c_void_p])
 _sig("atm_dg_nstates", c_int, [c_void_p])
 _sig("atm_dg_nnodes", c_int64, [c_void_p])
 _sig("atm_dg_get_epw", c_int, [c_void_p])
+_sig("atm_dg_set_schedule", c_int, [c_void_p, ctypes.POINTER(ctypes.c_int32), c_int64])
 _sig("atm_dg_set_epw", c_int, [c_void_p, c_int])
 _sig("atm_dg_epw_candidates", c_int, [c_void_p, ctypes.POINTER(ctypes.c_int32), c_int])
 _sig("atm_dg_autotune", c_int, [c_void_p, c_void_p, c_void_p, c_int, ctypes.POINTER(ctypes.c_int32),

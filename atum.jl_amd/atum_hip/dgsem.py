@@ -54,13 +54,19 @@ class FluxDifferencingForm:
 
 class DGSEM:
     def __init__(self, *, law, grid, surface_numericalflux, volume_form=None, directions=None,
-                 auxstate=None, precision="reference", geometry="host", ctx=None):
+                 auxstate=None, precision="reference", geometry="host", ctx=None, element_order="id"):
         """precision: "reference" keeps the reference's summation order with every
         product and sum rounded; "fast" evaluates symmetric flux pairs once with FMA
         (same operator, round-off-level differences).
         geometry: "host" copies the grid's geometry arrays to the device; "device"
         has the library build them from the brick's vertex coordinates
-        (atm_dg_create_brick), bit-identical, without forming them on the host."""
+        (atm_dg_create_brick), bit-identical, without forming them on the host.
+        element_order: "id" (default) has the element kernels take the elements in
+        element-id order, "morton" in Morton order of the brick (atm_dg_set_schedule);
+        results are the same either way (Morton measured slower: the indirection adds
+        a dependent load before each element's first load)."""
+        if element_order not in ("morton", "id"):
+            raise ValueError("element_order must be 'morton' or 'id'")
         self.ctx = ctx or _lib.context()
         self.law = law
         self.grid = grid
@@ -141,6 +147,19 @@ class DGSEM:
                            "atm_dg_create")
         self.h = h
         self.Ns = _lib.lib.atm_dg_nstates(h)
+        self.element_order = element_order
+        if element_order == "morton" and grid.K > 1:
+            self.set_schedule(grid.morton_order())
+
+    def set_schedule(self, order=None):
+        """Processing order of the elements (a permutation of the element ids; None:
+        element-id order).  Results do not depend on it."""
+        if order is None:
+            self.ctx.check(_lib.lib.atm_dg_set_schedule(self.h, None, 0), "atm_dg_set_schedule")
+            return
+        o = np.ascontiguousarray(order, dtype=np.int32)
+        self.ctx.check(_lib.lib.atm_dg_set_schedule(self.h, o.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)),
+                                                    int(o.size)), "atm_dg_set_schedule")
 
     # src/dgsem.jl:41-51: MJ, MJI, faceMJ (downloaded once when built on the device)
     @property

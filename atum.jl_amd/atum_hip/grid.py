@@ -232,6 +232,17 @@ class Grid:
         self.normal = self.normal.reshape(dim, -1)
         self.bctag = self.bctag.ravel()
 
+    def morton_order(self):
+        """Element ids in Morton (Z-curve) order of their brick multi-indices: the
+        processing order that keeps neighbouring elements close in time."""
+        em = self._elements()
+        key = np.zeros(self.K, dtype=np.int64)
+        bits = max(int(k - 1).bit_length() for k in self.Kd) if self.K > 0 else 0
+        for b in range(bits):
+            for d in range(self.dim):
+                key |= ((em[d] >> b) & 1) << (self.dim * b + d)
+        return np.argsort(key, kind="stable").astype(np.int32)
+
     def columns(self):
         """(colelem, KV, ncols): elements of each vertical column, bottom to top."""
         KV = self.Kd[-1]

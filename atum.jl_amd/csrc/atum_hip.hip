@@ -1341,6 +1341,7 @@ int atm_dg_destroy(atm_dg* dg) {
   hipSetDevice(dg->ctx->device);
   hipStreamSynchronize(stream_of(dg->ctx));
   if (dg->lsrk_exec) hipGraphExecDestroy(dg->lsrk_exec);
+  if (dg->sched) hipFree(dg->sched);
   for (void* p : dg->allocs) hipFree(p);
   delete dg;
   return ATM_OK;
@@ -1365,6 +1366,39 @@ int atm_dg_set_epw(atm_dg* dg, int epw) {
   dg->lds = lds;
   dg->dev.epw = epw;
   dg->dev.lds = (int)lds;
+  return ATM_OK;
+}
+
+int atm_dg_set_schedule(atm_dg* dg, const int32_t* order, int64_t n) {
+  if (!dg) return ATM_ERR_ARG;
+  atm_ctx* ctx = dg->ctx;
+  const int64_t K = dg->dev.K;
+  if (order && n != K) return fail(ctx, ATM_ERR_ARG, "atm_dg_set_schedule: the order must list every element");
+  int32_t* dev = nullptr;
+  if (order && K > 0) {
+    std::vector<uint8_t> seen((size_t)K, 0);
+    for (int64_t i = 0; i < K; ++i) {
+      const int64_t v = order[i];
+      if (v < 0 || v >= K || seen[(size_t)v])
+        return fail(ctx, ATM_ERR_ARG, "atm_dg_set_schedule: the order is not a permutation of the elements");
+      seen[(size_t)v] = 1;
+    }
+    hipSetDevice(ctx->device);
+    HIPCHK(ctx, hipMalloc(&dev, (size_t)K * sizeof(int32_t)));
+    hipError_t e = hipMemcpy(dev, order, (size_t)K * sizeof(int32_t), hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+      hipFree(dev);
+      return hipfail(ctx, e, "atm_dg_set_schedule upload");
+    }
+  }
+  hipSetDevice(ctx->device);
+  HIPCHK(ctx, hipStreamSynchronize(stream_of(ctx)));  // launches in flight may read the old order
+  if (dg->lsrk_exec) hipGraphExecDestroy(dg->lsrk_exec);  // captured with the old order
+  dg->lsrk_exec = nullptr;
+  dg->lsrk_scheme = -1;
+  if (dg->sched) hipFree(dg->sched);
+  dg->sched = dev;
+  dg->dev.sched = dev;
   return ATM_OK;
 }
 

@@ -23,6 +23,7 @@ struct DevDG {
   int K, Nq, Np, Nfpf, Nfp, naux, dirmask, vflux, sflux;
   int64_t Ntot, NF;
   int epw, lds;     // elements per workgroup and LDS bytes of a single-operator launch
+  const int32_t* sched;  // processing order of the elements (null: element-id order)
   const double* D;  // Nq*Nq column major
   const double* g;
   const double* MJ;
@@ -221,8 +222,9 @@ ATM_DEV void fd_body(const DevDG& p, double* lds, int64_t blk, const double* __r
   const int tid = threadIdx.x;
   const int el = tid / Np;
   const int ijk = tid - el * Np;
-  const int64_t e = blk * epw + el;
-  const bool active = el < epw && e < p.K;
+  const int64_t le = blk * epw + el;  // position in the processing order
+  const bool active = el < epw && le < p.K;
+  const int64_t e = (p.sched && active) ? (int64_t)p.sched[le] : le;
   double* sP = lds + Nq * Nq + el * stride;  // [k][node]
   double* slg = sP + NP * Np;                // [d][node], current direction
   for (int t = tid; t < Nq * Nq; t += blockDim.x) sD[t] = p.D[t];
@@ -322,8 +324,9 @@ ATM_DEV void weak_body(const DevDG& p, double* lds, int64_t blk, const double* _
   const int tid = threadIdx.x;
   const int el = tid / Np;
   const int ijk = tid - el * Np;
-  const int64_t e = blk * epw + el;
-  const bool active = el < epw && e < p.K;
+  const int64_t le = blk * epw + el;  // position in the processing order
+  const bool active = el < epw && le < p.K;
+  const int64_t e = (p.sched && active) ? (int64_t)p.sched[le] : le;
   double* slF = lds + Nq * Nq + el * stride;  // [dir slot][s][node]
   for (int t = tid; t < Nq * Nq; t += blockDim.x) sD[t] = p.D[t];
   double q1[Ns], a1[NA];

@@ -176,8 +176,9 @@ ATM_DEV void fd_fast_body(const DevDG& p, double* lds, int64_t blk, const double
   const int tid = threadIdx.x;
   const int el = tid / Np;
   const int ijk = tid - el * Np;
-  const int64_t e = blk * epw + el;
-  const bool active = el < epw && e < p.K;
+  const int64_t le = blk * epw + el;  // position in the processing order
+  const bool active = el < epw && le < p.K;
+  const int64_t e = (p.sched && active) ? (int64_t)p.sched[le] : le;
   double* sP = lds + Nq * Nq + el * stride;  // [k][node]
   double* slg = sP + NP * Np;                // [d][node]
   double* sx = slg + D * Np;                 // [m][s][node]

@@ -171,6 +171,11 @@ int atm_dg_get_epw(const atm_dg* dg);
 int atm_dg_set_epw(atm_dg* dg, int epw);
 int atm_dg_epw_candidates(const atm_dg* dg, int32_t* out, int max);
 int atm_dg_autotune(atm_dg* dg, const double* q, double* dq, int reps, int32_t* epw_out, float* us_out, int max);
+/* Order in which the element kernels take the elements (order[i] = the element
+ * processed i-th, a permutation of 0..K-1; NULL: element-id order).  Results do
+ * not depend on it; a compact order (e.g. Morton) keeps a workgroup's face
+ * neighbours in the same L2.  No counterpart in the reference. */
+int atm_dg_set_schedule(atm_dg* dg, const int32_t* order, int64_t n);
 /* total nodes Np*K */
 int64_t atm_dg_nnodes(const atm_dg* dg);
 

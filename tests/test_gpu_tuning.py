@@ -125,3 +125,33 @@ def test_autotune_batch_groups():
         A.LSRK54(d, alone, dt).run(alone, 4)
         d.ctx.sync()
         np.testing.assert_array_equal(q.numpy(), alone.numpy())
+
+
+@pytest.mark.parametrize("cfg", [CONFIGS[1], CONFIGS[2], CONFIGS[4], CONFIGS[5]])
+def test_element_order_does_not_change_results(cfg):
+    """The processing order (atm_dg_set_schedule): element-id order, Morton order and
+    a random permutation give bit-identical tendencies and steps; an order that is
+    not a permutation is refused."""
+    import atum_hip as A
+    dg, q0 = _dg(*cfg)
+    q = dg.fieldarray(q0)
+    rng = np.random.default_rng(1)
+    ref = None
+    for order in (None, dg.grid.morton_order(), rng.permutation(dg.grid.K)):
+        dg.set_schedule(order)
+        dq = dg.fieldarray()
+        dg(dq, q, 0.0, increment=False)
+        qs = dg.fieldarray(q0)
+        A.LSRK54(dg, qs, 1e-4).run(qs, 3)
+        dg.ctx.sync()
+        got = (dq.numpy(), qs.numpy())
+        if ref is None:
+            ref = got
+        np.testing.assert_array_equal(got[0], ref[0])
+        np.testing.assert_array_equal(got[1], ref[1])
+    bad = np.arange(dg.grid.K)
+    bad[-1] = 0
+    with pytest.raises(A.AtumError):
+        dg.set_schedule(bad)
+    with pytest.raises(A.AtumError):
+        dg.set_schedule(np.arange(dg.grid.K - 1))
