@@ -165,8 +165,11 @@ int atm_dg_nstates(const atm_dg* dg);
  * their count.  atm_dg_autotune times `reps` tendency launches dq = L(q) per
  * candidate with HIP events on the context's stream, keeps the fastest and
  * returns it (per-candidate epw and microseconds per launch go to the optional
- * arrays of capacity max); dq is overwritten.  No counterpart in the reference
- * (KernelAbstractions workgroup sizes are fixed in src/dgsem.jl:74-76). */
+ * arrays of capacity max); dq is overwritten.  atm_dg_set_epw waits for the
+ * context's stream; it must not be called while other streams (a serving
+ * scheduler holding requests of this operator) may launch the operator.  No
+ * counterpart in the reference (KernelAbstractions workgroup sizes are fixed in
+ * src/dgsem.jl:74-76). */
 int atm_dg_get_epw(const atm_dg* dg);
 int atm_dg_set_epw(atm_dg* dg, int epw);
 int atm_dg_epw_candidates(const atm_dg* dg, int32_t* out, int max);
@@ -174,7 +177,9 @@ int atm_dg_autotune(atm_dg* dg, const double* q, double* dq, int reps, int32_t* 
 /* Order in which the element kernels take the elements (order[i] = the element
  * processed i-th, a permutation of 0..K-1; NULL: element-id order).  Results do
  * not depend on it; a compact order (e.g. Morton) keeps a workgroup's face
- * neighbours in the same L2.  No counterpart in the reference. */
+ * neighbours in the same L2 (measured slower than element-id order on the
+ * brick meshes here).  Same threading rule as atm_dg_set_epw.  No counterpart in
+ * the reference. */
 int atm_dg_set_schedule(atm_dg* dg, const int32_t* order, int64_t n);
 /* total nodes Np*K */
 int64_t atm_dg_nnodes(const atm_dg* dg);
