@@ -310,7 +310,8 @@ typedef struct {
 } atm_server_stats;
 
 int atm_server_create(atm_ctx* ctx, const atm_server_desc* desc, atm_server** out);
-/* finishes every submitted request, then stops the worker */
+/* finishes every submitted request, then stops the worker; no other thread may
+ * be inside a call on this server (e.g. blocked in atm_server_wait) */
 int atm_server_destroy(atm_server* s);
 /* copies q0; validates the request (operators of this context, footprint within
  * the budget); *id identifies it from then on */
