@@ -360,6 +360,11 @@ ATM_DEV void weak_body(const DevDG& p, double* lds, int64_t blk, const double* _
       }
     }
   }
+  // epilogue operands loaded before the barrier (their latency overlaps the volume term)
+  double dq0[Ns];
+  const double MJIn = active ? p.MJI[node] : 0.0;
+#pragma unroll
+  for (int s = 0; s < Ns; ++s) dq0[s] = (active && !assign) ? dq[s * p.Ntot + node] : 0.0;
   __syncthreads();
   if (!active) return;
   int idx[3];
@@ -368,7 +373,6 @@ ATM_DEV void weak_body(const DevDG& p, double* lds, int64_t blk, const double* _
 #pragma unroll
   for (int s = 0; s < Ns; ++s) dl[s] = -0.0;
   L::sources(p.c, dl, q1, a1, p.dirmask, true);
-  const double MJIn = p.MJI[node];
   const int i = idx[0], j = idx[1], k = idx[2];
   for (int n = 0; n < Nq; ++n) {
     const double Dni = sD[n + Nq * i] * MJIn;
@@ -392,7 +396,7 @@ ATM_DEV void weak_body(const DevDG& p, double* lds, int64_t blk, const double* _
   }
   double acc[Ns];
 #pragma unroll
-  for (int s = 0; s < Ns; ++s) acc[s] = assign ? dl[s] : alpha * dq[s * p.Ntot + node] + dl[s];
+  for (int s = 0; s < Ns; ++s) acc[s] = assign ? dl[s] : alpha * dq0[s] + dl[s];
   surface_node<L, SF>(p, Nq, e, idx, q, q1, a1, MJIn, acc);
   epilogue<L>(p, node, q1, acc, dq, qout, bdt);
 }

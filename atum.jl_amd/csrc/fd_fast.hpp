@@ -195,6 +195,11 @@ ATM_DEV void fd_fast_body(const DevDG& p, double* lds, int64_t blk, const double
     for (int k = 0; k < NP; ++k) sP[k * Np + ijk] = P1[k];
     MJn = p.MJ[node];
   }
+  // epilogue operands loaded with the state (their latency overlaps the volume term)
+  double dq0[Ns];
+  const double MJIn = active ? p.MJI[node] : 0.0;
+#pragma unroll
+  for (int s = 0; s < Ns; ++s) dq0[s] = (active && !assign) ? dq[s * p.Ntot + node] : 0.0;
   // metric rows of every direction, loaded with the state (one memory latency)
   double gall[D][D];
 #pragma unroll
@@ -274,12 +279,11 @@ ATM_DEV void fd_fast_body(const DevDG& p, double* lds, int64_t blk, const double
   if (active) surface_faces_lds<L, SF, true>(p, Nq, Np, e, ijk, q, sx);
   __syncthreads();
   if (!active) return;
-  const double MJIn = p.MJI[node];
   double acc[Ns];
 #pragma unroll
   for (int s = 0; s < Ns; ++s) {
     const double v = src[s] + MJIn * vol[s];
-    acc[s] = assign ? v : alpha * dq[s * p.Ntot + node] + v;
+    acc[s] = assign ? v : alpha * dq0[s] + v;
   }
   surface_gather<L>(p, Nq, idx, sx, acc);
   epilogue<L>(p, node, q1, acc, dq, qout, bdt);
