@@ -250,6 +250,10 @@ ATM_DEV void fd_body(const DevDG& p, double* lds, int64_t blk, const double* __r
   int idx[3];
   node_index<D>(ijk, Nq, idx);
   const double MJIn = active ? p.MJI[node] : 0.0;
+  // the stage's dq, loaded with the state (its latency overlaps the volume term)
+  double dq0[Ns];
+#pragma unroll
+  for (int s = 0; s < Ns; ++s) dq0[s] = (active && !assign) ? dq[s * p.Ntot + node] : 0.0;
   double acc[Ns];
   bool first = true;
 #pragma unroll
@@ -295,7 +299,7 @@ ATM_DEV void fd_body(const DevDG& p, double* lds, int64_t blk, const double* __r
     }
     if (first) {
 #pragma unroll
-      for (int s = 0; s < Ns; ++s) acc[s] = assign ? dl[s] : alpha * dq[s * p.Ntot + node] + dl[s];
+      for (int s = 0; s < Ns; ++s) acc[s] = assign ? dl[s] : alpha * dq0[s] + dl[s];
       first = false;
     } else {
 #pragma unroll
@@ -305,7 +309,7 @@ ATM_DEV void fd_body(const DevDG& p, double* lds, int64_t blk, const double* __r
   if (!active) return;
   if (first) {
 #pragma unroll
-    for (int s = 0; s < Ns; ++s) acc[s] = assign ? 0.0 : alpha * dq[s * p.Ntot + node];
+    for (int s = 0; s < Ns; ++s) acc[s] = assign ? 0.0 : alpha * dq0[s];
   }
   surface_node<L, SF>(p, Nq, e, idx, q, q1, a1, MJIn, acc);
   epilogue<L>(p, node, q1, acc, dq, qout, bdt);
