@@ -9,6 +9,9 @@ A ring of `ring` buffers bounds the memory and the copies in flight.
 
 File format (little-endian): b"ATMSNAP1", uint64 header length, a JSON header
 (shape, dtype, step, time and caller metadata), then the array bytes in C order.
+With fmt="vtk" the writer thread writes each snapshot as a .vtu instead (the
+fields interpolated to equally spaced points, as the reference's experiments
+write them, `vtk.py`) and `close()` writes a .pvd collection of the run.
 """
 from __future__ import annotations
 
@@ -21,6 +24,7 @@ import threading
 import numpy as np
 
 from . import _lib
+from . import vtk as _vtk
 from .fields import Field
 
 MAGIC = b"ATMSNAP1"
@@ -96,11 +100,25 @@ class SnapshotWriter:
     pattern.format(step=..., time=...) without stalling the device."""
 
     def __init__(self, q: Field, pattern: str, every: int = 1, ring: int = 2, meta=None, first_slot=0,
-                 solver=None):
+                 solver=None, fmt: str = "atm", grid=None, names=None, reference=None, pvd: str | None = None):
         """solver: if given, the header's time is solver.time when the hook runs (the
-        time of the state written); otherwise the time the hook receives."""
+        time of the state written); otherwise the time the hook receives.
+        fmt="vtk": write .vtu files; needs `grid` (node coordinates `points`, N, dim) and
+        takes field `names` (default q1, q2, ...), an optional `reference` state
+        subtracted before writing (as the reference's experiments write q - q_ref) and
+        the `pvd` collection path written by close()."""
         if not 1 <= ring <= 16 or first_slot < 0 or first_slot + ring > 16:
             raise ValueError("ring slots must lie in [0, 16)")
+        if fmt not in ("atm", "vtk"):
+            raise ValueError("fmt must be 'atm' or 'vtk'")
+        if fmt == "vtk" and grid is None:
+            raise ValueError("VTK output needs the grid")
+        self.fmt, self.grid, self.pvd = fmt, grid, pvd
+        self.names = list(names) if names is not None else [f"q{s + 1}" for s in range(q.shape[0])]
+        if len(self.names) != q.shape[0]:
+            raise ValueError("one name per state component")
+        self.reference = None if reference is None else np.asarray(reference, dtype=np.float64)
+        self.pvd_entries = []
         self.ctx = q.ctx
         self.pattern, self.every = pattern, int(every)
         self.meta = dict(meta or {})
@@ -126,7 +144,14 @@ class SnapshotWriter:
             i, path, head = task
             try:
                 self.ctx.check(_lib.lib.atm_snapshot_wait(self.ctx.h, self.slots[i]), "atm_snapshot_wait")
-                write_snapshot(path, self.host[i].array, head)
+                if self.fmt == "vtk":
+                    a = self.host[i].array
+                    if self.reference is not None:
+                        a = a - self.reference
+                    _vtk.write_vtu(path, self.grid.points, dict(zip(self.names, a)), self.grid.N, self.grid.dim)
+                    self.pvd_entries.append((head["time"], path))
+                else:
+                    write_snapshot(path, self.host[i].array, head)
                 self.files.append(path)
             except Exception as exc:  # reported by close()
                 self.errors.append(exc)
@@ -152,4 +177,6 @@ class SnapshotWriter:
         self._thread.join()
         if self.errors:
             raise self.errors[0]
+        if self.fmt == "vtk" and self.pvd:
+            _vtk.write_pvd(self.pvd, self.pvd_entries)
         return list(self.files)

@@ -86,3 +86,29 @@ def test_checkpoint_restart_lsrk_and_ark(tmp_path):
         make(d, q2, t0).run(q2, 5)
         d.ctx.sync()
         np.testing.assert_array_equal(q2.numpy(), full.numpy(), err_msg=name)
+
+
+def test_vtk_snapshots(tmp_path):
+    """fmt="vtk": the snapshot stream writes .vtu files of q - q_ref (interpolated to
+    equally spaced points, as the reference's experiments write) and a .pvd of the run."""
+    import xml.etree.ElementTree as ET
+    import atum_hip as A
+    from atum_hip import vtk
+    dg, q0, dt = _vortex()
+    q = dg.fieldarray(q0)
+    solver = A.LSRK54(dg, q, dt)
+    ref = np.ones_like(q0)
+    w = A.SnapshotWriter(q, str(tmp_path / "v_{step:03d}.vtu"), every=3, solver=solver, fmt="vtk", grid=dg.grid,
+                         names=["rho", "rhou", "rhov", "rhoe"], reference=ref, pvd=str(tmp_path / "v.pvd"))
+    for step in range(1, 7):
+        t = solver.time
+        solver.dostep(q)
+        w(step, t, q)
+    files = w.close()
+    assert len(files) == 2
+    last = ET.parse(sorted(files)[-1]).getroot()
+    got = {da.get("Name"): np.array(da.text.split(), dtype=float) for da in last.iter("DataArray") if da.get("Name")}
+    want = vtk.to_equispaced((q.numpy()[0] - 1.0).reshape(dg.grid.Np, dg.grid.K, order="F"), dg.grid.N, 2)
+    np.testing.assert_array_equal(got["rho"], want.reshape(-1, order="F"))
+    pvd = ET.parse(str(tmp_path / "v.pvd")).getroot().findall("Collection/DataSet")
+    assert [float(d.get("timestep")) for d in pvd] == pytest.approx([3 * dt, 6 * dt], rel=1e-12)
