@@ -236,7 +236,11 @@ def weak_workload(args):
     dg = A.DGSEM(law=A.ShallowWaterLaw(2, grav=9.81), grid=grid, surface_numericalflux=A.RusanovFlux(),
                  geometry="device")
     q0 = P.sw_bump(grid.points, 0.05, 0.1, (0.5, 0.5))
-    tuned = A.autotune(dg, q0, reps=20) if args.tune else None
+    tuned = None
+    if args.epw:
+        dg.epw = args.epw
+    elif args.tune:
+        tuned = A.autotune(dg, q0, reps=20)
     dt = P.sw_ensemble_dt(K1, N, 0.05)
     q = dg.fieldarray(q0)
     s_ = A.LSRK54(dg, q, dt)
@@ -412,6 +416,8 @@ def main():
     ap.add_argument("--weak-K", type=int, default=256, help="weak workload: elements per direction")
     ap.add_argument("--serve-seconds", type=float, default=10.0)
     ap.add_argument("--chunk", type=int, default=10, help="steps per serving tick")
+    ap.add_argument("--epw", type=int, default=0,
+                    help="bubble / weak: use this many elements per workgroup instead of tuning")
     ap.add_argument("--no-tune", dest="tune", action="store_false",
                     help="bubble: keep the default elements per workgroup (no autotuning before timing)")
     ap.add_argument("--ensemble-tune", action="store_true",
@@ -448,7 +454,9 @@ def main():
     def timed(precision):
         grid, dg, q0, dt = bubble_setup(Kd, N, precision)
         ctx = dg.ctx
-        if args.tune:  # elements per workgroup chosen by timing, before the timed region
+        if args.epw and precision == args.precision:  # a fixed workgroup shape (e.g. for counter runs)
+            dg.epw = args.epw
+        elif args.tune:  # elements per workgroup chosen by timing, before the timed region
             best, times = A.autotune(dg, q0, reps=50)
             tuning[precision] = {"epw": best, "us_per_launch": times}
         q = dg.fieldarray(q0)

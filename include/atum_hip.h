@@ -163,7 +163,8 @@ int atm_dg_nstates(const atm_dg* dg);
  * computed by its own threads in the same order).  atm_dg_epw_candidates lists
  * the shapes that fit (whole waves, at most 512 threads, the LDS); it returns
  * their count.  atm_dg_autotune times `reps` tendency launches dq = L(q) per
- * candidate with HIP events on the context's stream, keeps the fastest and
+ * candidate with HIP events on the context's stream, keeps the fastest (the
+ * operator's current shape unless another is more than 2 % faster) and
  * returns it (per-candidate epw and microseconds per launch go to the optional
  * arrays of capacity max); dq is overwritten.  atm_dg_set_epw waits for the
  * context's stream; it must not be called while other streams (a serving

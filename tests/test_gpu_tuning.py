@@ -64,12 +64,15 @@ def test_invalid_shape_refused():
 def test_autotune_and_cache(tmp_path):
     import atum_hip as A
     dg, q0 = _dg("euler", 2, 4, 8, "ec", "rusanov", "fast")
+    before = dg.epw
     path = str(tmp_path / "tune.json")
     cache = A.TuneCache(path)
     best, times = A.autotune(dg, q0, reps=5, cache=cache)
     assert best in dg.epw_candidates() and dg.epw == best
     assert sorted(times) == sorted(dg.epw_candidates()) and all(t > 0 for t in times.values())
-    assert best == min(times, key=times.get)
+    fastest = min(times, key=times.get)
+    # the previous shape is kept unless another is more than 2 % faster
+    assert best == fastest or (best == before and times[fastest] > 0.98 * times[before])
     stored = json.load(open(path))
     assert list(stored.values()) == [best]
     # a second operator of the same kind takes the stored choice without timing
