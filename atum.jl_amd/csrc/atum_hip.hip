@@ -731,9 +731,25 @@ band_solve_mfma_kernel(BandGeo g, const double* __restrict__ frags, const int32_
   double v[KS];
   double f0[2 * NF], f1[2 * NF], f2[2 * NF];  // fragments of three consecutive blocks (fixed buffers)
   double r0[KS], r1[KS], r2[KS];              // right-hand-side rows of the same blocks (forward pass)
+  // Tiles (16 rows x 4 columns) that are identically zero: with BS = bw + 1 the
+  // coupling blocks are strictly triangular (L_{i,i-1} strictly upper, U_{i,i+1}
+  // strictly lower) and the diagonal inverses triangular (L_ii^-1 lower, U_ii^-1
+  // upper), so some tiles hold only exact zeros; their MFMAs would add +0 and are
+  // skipped together with their fragment loads.  mi: 0 -L_{i,i-1}, 1 L_ii^-1,
+  // 2 -U_{i,i+1}, 3 U_ii^-1.
+  auto tile_used = [](int mi, int ti, int ks) -> bool {
+    const int r0 = 16 * ti, r1 = (16 * ti + 15 < BS - 1) ? 16 * ti + 15 : BS - 1;
+    const int c0 = 4 * ks, c1 = (4 * ks + 3 < BS - 1) ? 4 * ks + 3 : BS - 1;
+    if (mi == 0) return r0 < c1;   // some r < c
+    if (mi == 1) return r1 >= c0;  // some r >= c
+    if (mi == 2) return r1 > c0;   // some r > c
+    return r0 <= c1;               // some r <= c
+  };
   auto fetch = [&](int ez, int m0, double* dst) {
 #pragma unroll
-    for (int f = 0; f < 2 * NF; ++f) dst[f] = Fu[((int64_t)(ez * 4 + m0) * NF + f) * 64 + l];
+    for (int f = 0; f < 2 * NF; ++f)
+      if (tile_used(m0 + f / NF, (f % NF) / KS, (f % NF) % KS))
+        dst[f] = Fu[((int64_t)(ez * 4 + m0) * NF + f) * 64 + l];
   };
   auto fetch_rows = [&](int ez, double* dst) {
 #pragma unroll
@@ -767,6 +783,7 @@ band_solve_mfma_kernel(BandGeo g, const double* __restrict__ frags, const int32_
       for (int ks = 0; ks < KS; ++ks)
 #pragma unroll
         for (int ti = 0; ti < TI; ++ti) {
+          if (!tile_used(2 * pass, ti, ks)) continue;
           if (ks % 2 == 0) a0[ti] = __builtin_amdgcn_mfma_f64_16x16x4f64(fa[ti * KS + ks], v[ks], a0[ti], 0, 0, 0);
           else a1[ti] = __builtin_amdgcn_mfma_f64_16x16x4f64(fa[ti * KS + ks], v[ks], a1[ti], 0, 0, 0);
         }
@@ -789,6 +806,7 @@ band_solve_mfma_kernel(BandGeo g, const double* __restrict__ frags, const int32_
       for (int ks = 0; ks < KS; ++ks)
 #pragma unroll
         for (int ti = 0; ti < TI; ++ti) {
+          if (!tile_used(2 * pass + 1, ti, ks)) continue;
           if (ks % 2 == 0) a0[ti] = __builtin_amdgcn_mfma_f64_16x16x4f64(fa[NF + ti * KS + ks], t[ks], a0[ti], 0, 0, 0);
           else a1[ti] = __builtin_amdgcn_mfma_f64_16x16x4f64(fa[NF + ti * KS + ks], t[ks], a1[ti], 0, 0, 0);
         }
