@@ -3,6 +3,11 @@ import sys
 
 import pytest
 
+# The oracle library is OpenMP-parallel; under pytest-xdist every worker would
+# start a full team of spinning threads (a 30-minute suite instead of seconds).
+if "PYTEST_XDIST_WORKER" in os.environ:
+    os.environ.setdefault("OMP_NUM_THREADS", "1")
+
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 for p in (ROOT, os.path.join(ROOT, "tests"), os.path.join(ROOT, "atum.jl_amd")):
     if p not in sys.path:
