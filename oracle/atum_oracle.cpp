@@ -19,6 +19,15 @@
 // Bennu (external, un-vendored, compat 0.3) pieces are reconstructed from their
 // call sites: LobattoCell operators, brickgrid (Cartesian and stacked
 // ordering), affine metrics, face tables, banded LU and solve.
+//
+// Pinning: the reference tests' golden values, closed forms and invariants
+// (SBP, free stream, conservation, entropy, convergence orders, dense-Jacobian
+// and dense-solve checks) pin every function here except three details, which
+// are parity unpinned: the bitwise operation order of Bennu's banded LU (the LU
+// itself is pinned against a dense solve), the point order inside a face (pinned
+// only by the face-table consistency checks), and the iteration path of the
+// relaxation solve (Newton on the reference's scalar equation; the reference
+// uses NLsolve, so only the converged factor is comparable).
 #include <omp.h>
 
 #include <algorithm>
