@@ -401,7 +401,10 @@ def main():
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--elements", type=int, default=16, help="elements per direction")
     ap.add_argument("--N", type=int, default=4)
-    ap.add_argument("--cpu-stages", type=int, default=150)
+    ap.add_argument("--cpu-stages", type=int, default=450)
+    ap.add_argument("--dt-scale", type=float, default=1.0,
+                    help="bubble: multiply the rising-bubble time step (its CFL formula is the "
+                         "reference's for N = 4; higher degrees need a smaller step to stay stable)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-graph", action="store_true")
     ap.add_argument("--precision", choices=["fast", "reference"], default="fast",
@@ -453,6 +456,7 @@ def main():
 
     def timed(precision):
         grid, dg, q0, dt = bubble_setup(Kd, N, precision)
+        dt *= args.dt_scale
         ctx = dg.ctx
         if args.epw and precision == args.precision:  # a fixed workgroup shape (e.g. for counter runs)
             dg.epw = args.epw
