@@ -206,7 +206,8 @@ class DGSEM:
         self.ctx.check(_lib.lib.atm_dg_set_epw(self.h, int(value)), "atm_dg_set_epw")
 
     def epw_candidates(self):
-        """The workgroup shapes that fit (whole waves, <= 512 threads, the device's LDS)."""
+        """The workgroup shapes that fit (whole waves, <= 512 threads or 768 for the FAST
+        kernel's 3-D N = 6 elements with the Roe or Rusanov surface flux, the device's LDS)."""
         out = (ctypes.c_int32 * 8)()
         n = _lib.lib.atm_dg_epw_candidates(self.h, out, 8)
         return [int(out[i]) for i in range(n)]

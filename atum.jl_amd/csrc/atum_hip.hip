@@ -1259,6 +1259,7 @@ int dg_prepare(atm_ctx* ctx, const atm_dg_desc* d, atm_dg** out) {
   dg->epw = r.epw;
   dg->lds_fixed = r.lds_fixed;
   dg->lds_elem = r.lds_elem;
+  dg->max_threads = r.max_threads;
   dg->mfn = resolve_measures(d->law, d->dim);
   DevDG& p = dg->dev;
   p.epw = r.epw;
@@ -1372,8 +1373,9 @@ int atm_dg_get_epw(const atm_dg* dg) { return dg ? dg->epw : ATM_ERR_ARG; }
 int atm_dg_set_epw(atm_dg* dg, int epw) {
   if (!dg) return ATM_ERR_ARG;
   const size_t lds = dg->lds_fixed + (size_t)(epw > 0 ? epw : 0) * dg->lds_elem;
-  if (epw < 1 || epw > 512 || epw * dg->Np > 512 || lds > dg->ctx->max_lds)
-    return fail(dg->ctx, ATM_ERR_ARG, "atm_dg_set_epw: the workgroup would exceed 512 threads or the LDS");
+  if (epw < 1 || epw > 512 || epw * dg->Np > dg->max_threads || lds > dg->ctx->max_lds)
+    return fail(dg->ctx, ATM_ERR_ARG,
+                "atm_dg_set_epw: the workgroup would exceed the kernel's thread limit (atm_dg_epw_candidates) or the LDS");
   if (epw == dg->epw) return ATM_OK;
   hipSetDevice(dg->ctx->device);
   HIPCHK(dg->ctx, hipStreamSynchronize(stream_of(dg->ctx)));
@@ -1426,7 +1428,7 @@ int atm_dg_epw_candidates(const atm_dg* dg, int32_t* out, int max) {
   // 64/128/256/512 lanes of small elements otherwise
   int cand[8], n = 0;
   auto add = [&](int e) {
-    if (e < 1 || e * dg->Np > 512 || dg->lds_fixed + (size_t)e * dg->lds_elem > dg->ctx->max_lds) return;
+    if (e < 1 || e * dg->Np > dg->max_threads || dg->lds_fixed + (size_t)e * dg->lds_elem > dg->ctx->max_lds) return;
     for (int i = 0; i < n; ++i)
       if (cand[i] == e) return;
     cand[n++] = e;

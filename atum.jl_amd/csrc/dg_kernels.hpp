@@ -535,10 +535,12 @@ struct Resolved {
   BatchLaunchFn bfn;
   int epw;     // default elements per workgroup
   size_t lds_fixed, lds_elem;
+  int max_threads = 512;  // largest workgroup the kernels are built for
 };
 
-inline Resolved make_resolved(LaunchFn fn, BatchLaunchFn bfn, int ns, int epw, size_t fixed, size_t elem) {
-  return Resolved{fn, fixed + (size_t)epw * elem, ns, bfn, epw, fixed, elem};
+inline Resolved make_resolved(LaunchFn fn, BatchLaunchFn bfn, int ns, int epw, size_t fixed, size_t elem,
+                              int max_threads = 512) {
+  return Resolved{fn, fixed + (size_t)epw * elem, ns, bfn, epw, fixed, elem, max_threads};
 }
 
 template <class L, int NQ, int VF, int SF>

@@ -289,11 +289,44 @@ ATM_DEV void fd_fast_body(const DevDG& p, double* lds, int64_t blk, const double
   epilogue<L>(p, node, q1, acc, dq, qout, bdt);
 }
 
-ATM_KERNEL_PAIR(fd_fast_kernel, fd_fast_body, class L ATM_COMMA int NQ ATM_COMMA int SF, L ATM_COMMA NQ ATM_COMMA SF)
+// Largest workgroup of the kernel.  An element of 5 to 7 waves (343 nodes at
+// N = 6) leaves a CU with one such workgroup resident at 3 waves per SIMD;
+// two elements in one workgroup of up to 12 waves keep almost twice the waves
+// in flight (N = 6 bubble: 74 -> 58 us per stage).  Elsewhere 512 threads (the
+// compiler then budgets registers for 2 waves per SIMD).  The variant with the
+// runtime surface-flux switch (SF < 0) needs more registers than 3 waves per
+// SIMD leave and keeps 512.
+template <class L, int NQ, int SF>
+constexpr int fast_max_threads() {
+  constexpr int Np = L::D == 1 ? NQ : (L::D == 2 ? NQ * NQ : NQ * NQ * NQ);
+  constexpr int waves = (Np + 63) / 64;
+  return (SF >= 0 && waves > 4 && waves < 8) ? 768 : 512;
+}
 
-template <class L, int NQ>
+template <class L, int NQ, int SF>
+__global__ void __attribute__((amdgpu_flat_work_group_size(1, fast_max_threads<L, NQ, SF>())))
+fd_fast_kernel(const DevDG p, const double* __restrict__ q, double* __restrict__ dq, double alpha, int assign,
+               double* __restrict__ qout, double bdt, int epw) {
+  extern __shared__ double lds[];
+  fd_fast_body<L, NQ, SF>(p, lds, xcd_block(), q, dq, alpha, assign, qout, bdt, epw);
+}
+
+template <class L, int NQ, int SF>
+__global__ void __attribute__((amdgpu_flat_work_group_size(1, fast_max_threads<L, NQ, SF>())))
+fd_fast_kernel_batch(const BatchEntry* __restrict__ tab, int n, int parity, double alpha, int assign, double b,
+                     int epw) {
+  extern __shared__ double lds[];
+  const int64_t lb = xcd_block();
+  const int i = batch_find(tab, n, lb);
+  const DevDG p = tab[i].p;
+  fd_fast_body<L, NQ, SF>(p, lds, lb - tab[i].block_start, tab[i].buf[parity], tab[i].dq, alpha, assign,
+                          tab[i].buf[1 - parity], b * tab[i].dt, epw);
+}
+
+template <class L, int NQ, int SF>
 constexpr int fast_epw() {
   constexpr int Np = L::D == 1 ? NQ : (L::D == 2 ? NQ * NQ : NQ * NQ * NQ);
+  if (fast_max_threads<L, NQ, SF>() > 512) return 2;
   return Np >= 64 ? 1 : 256 / Np;
 }
 
@@ -313,7 +346,7 @@ hipError_t launch_fd_fast(const DevDG& p, hipStream_t st, const double* q, doubl
   constexpr int Np = D == 1 ? NQ : (D == 2 ? NQ * NQ : NQ * NQ * NQ);
   const int epw = p.epw;
   const int threads = ((epw * Np + 63) / 64) * 64;
-  if (epw < 1 || threads > 512) return hipErrorInvalidConfiguration;
+  if (epw < 1 || threads > fast_max_threads<L, NQ, SF>()) return hipErrorInvalidConfiguration;
   const int blocks = (p.K + epw - 1) / epw;
   if (blocks == 0) return hipSuccess;
   const size_t lds = p.lds;

@@ -17,21 +17,17 @@ Resolved resolve_shallow_water(int dim, int form, int vflux, int sflux, int Nq, 
 Resolved resolve_fast_euler(int dim, int sflux, int Nq);
 Resolved resolve_fast_euler_gravity(int dim, int sflux, int Nq);
 
+template <class L, int NQ, int SF>
+Resolved fast_variant() {
+  return make_resolved(launch_fd_fast<L, NQ, SF>, launch_fd_fast_batch<L, NQ, SF>, L::Ns, fast_epw<L, NQ, SF>(),
+                       sizeof(double) * NQ * NQ, fast_lds_elem_bytes<L, NQ>(), fast_max_threads<L, NQ, SF>());
+}
+
 template <class L, int NQ>
 Resolved fast_entry(int sflux) {
-  Resolved r = make_resolved(nullptr, nullptr, L::Ns, fast_epw<L, NQ>(), sizeof(double) * NQ * NQ,
-                             fast_lds_elem_bytes<L, NQ>());
-  if (sflux == ATM_FLUX_ROE) {
-    r.fn = launch_fd_fast<L, NQ, ATM_FLUX_ROE>;
-    r.bfn = launch_fd_fast_batch<L, NQ, ATM_FLUX_ROE>;
-  } else if (sflux == ATM_FLUX_RUSANOV) {
-    r.fn = launch_fd_fast<L, NQ, ATM_FLUX_RUSANOV>;
-    r.bfn = launch_fd_fast_batch<L, NQ, ATM_FLUX_RUSANOV>;
-  } else {
-    r.fn = launch_fd_fast<L, NQ, -1>;
-    r.bfn = launch_fd_fast_batch<L, NQ, -1>;
-  }
-  return r;
+  if (sflux == ATM_FLUX_ROE) return fast_variant<L, NQ, ATM_FLUX_ROE>();
+  if (sflux == ATM_FLUX_RUSANOV) return fast_variant<L, NQ, ATM_FLUX_RUSANOV>();
+  return fast_variant<L, NQ, -1>();
 }
 
 template <template <int> class LAW, int D>

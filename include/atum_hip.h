@@ -161,8 +161,9 @@ int atm_dg_nstates(const atm_dg* dg);
  * fills 256 lanes with small elements and gives large ones a workgroup each.
  * atm_dg_set_epw changes it (results do not depend on it: every element is
  * computed by its own threads in the same order).  atm_dg_epw_candidates lists
- * the shapes that fit (whole waves, at most 512 threads, the LDS); it returns
- * their count.  atm_dg_autotune times `reps` tendency launches dq = L(q) per
+ * the shapes that fit (whole waves, at most 512 threads -- 768 for the FAST
+ * kernel's 3-D N = 6 elements with the Roe or Rusanov surface flux, whose
+ * default is two elements per workgroup -- and the LDS); it returns their count.  atm_dg_autotune times `reps` tendency launches dq = L(q) per
  * candidate with HIP events on the context's stream, keeps the fastest (the
  * operator's current shape unless another is more than 2 % faster) and
  * returns it (per-candidate epw and microseconds per launch go to the optional

@@ -15,7 +15,8 @@ pytestmark = pytest.mark.gpu
 CONFIGS = [("euler", 2, 3, 4, "ec", "rusanov", "reference"), ("euler", 2, 4, 3, "ec", "rusanov", "fast"),
            ("shallow_water", 2, 5, 3, "weak", "rusanov", "reference"), ("advection", 1, 3, 6, "central", "rusanov",
                                                                        "reference"),
-           ("euler_gravity", 3, 4, 2, "ec", "roe", "fast"), ("euler", 3, 2, 3, "weak", "roe", "reference")]
+           ("euler_gravity", 3, 4, 2, "ec", "roe", "fast"), ("euler", 3, 2, 3, "weak", "roe", "reference"),
+           ("euler_gravity", 3, 6, 2, "ec", "rusanov", "fast")]  # 768-thread workgroups of two elements
 
 
 def _dg(law, dim, N, K, form, sflux, precision):
@@ -103,6 +104,36 @@ def test_batch_members_with_different_shapes():
         A.LSRK54(d, alone, dt).run(alone, 6)
         d.ctx.sync()
         np.testing.assert_array_equal(q.numpy(), alone.numpy())
+
+
+def test_wide_workgroups_match_oracle_and_batch():
+    """The FAST kernel's 3-D N = 6 elements default to two per workgroup (704 threads);
+    one per workgroup and the batched kernel give the same bits, and the tendency
+    matches the oracle at 1e-12."""
+    import atum_hip as A
+    from gpu_cases import relmax
+    c = Case("euler_gravity", 3, 6, 2, form="ec", sflux="roe", bc="reflect", periodic=(False,) * 3,
+             gravity_domain=True, seed=91)
+    dg = A.DGSEM(law=c.dg.law, grid=c.grid, volume_form=c.dg.volume_form,
+                 surface_numericalflux=c.dg.surface_numericalflux, auxstate=c.aux, precision="fast")
+    assert dg.epw == 2 and dg.epw_candidates() == [1, 2]
+    q = dg.fieldarray(c.q)
+    dq = dg.fieldarray()
+    dg(dq, q, 0.0, increment=False)
+    dg.ctx.sync()
+    assert relmax(dq.numpy(), c.odg.tendency(c.q)) <= 1e-12
+    alone = dg.fieldarray(c.q)
+    A.LSRK54(dg, alone, 1e-2).run(alone, 4)
+    dg.ctx.sync()
+    b = A.Batch([dg])
+    qs, dqs, scr = [dg.fieldarray(c.q)], [dg.fieldarray()], [dg.fieldarray()]
+    b.lsrk_run(qs, dqs, scr, [1e-2], 4)
+    dg.epw = 1
+    one = dg.fieldarray(c.q)
+    A.LSRK54(dg, one, 1e-2).run(one, 4)
+    dg.ctx.sync()
+    np.testing.assert_array_equal(qs[0].numpy(), alone.numpy())
+    np.testing.assert_array_equal(one.numpy(), alone.numpy())
 
 
 def test_autotune_batch_groups():
