@@ -482,10 +482,15 @@ int atm_server_poll(atm_server* s, int64_t id) {
 int atm_server_wait(atm_server* s, int64_t id) {
   if (!s) return ATM_ERR_ARG;
   std::unique_lock<std::mutex> lk(s->mu);
-  Req* r = find(s, id);
-  if (!r) return fail(s->ctx, ATM_ERR_ARG, "atm_server_wait: unknown request");
-  s->cv_done.wait(lk, [r] { return finished_state(r->state); });
-  return r->state;
+  if (!find(s, id)) return fail(s->ctx, ATM_ERR_ARG, "atm_server_wait: unknown request");
+  // look the request up again at every wake-up: another thread may release it
+  // as soon as it has finished
+  s->cv_done.wait(lk, [s, id] {
+    const Req* x = find(s, id);
+    return !x || finished_state(x->state);
+  });
+  const Req* r = find(s, id);
+  return r ? r->state : ATM_REQ_DONE;
 }
 
 int atm_server_next_done(atm_server* s, double timeout_s, int64_t* id) {
@@ -522,10 +527,13 @@ int atm_server_times(atm_server* s, int64_t id, double* t) {
 }
 
 const char* atm_server_error(atm_server* s, int64_t id) {
+  // a copy per calling thread: the request may be released while the caller reads it
+  static thread_local std::string msg;
   if (!s) return "null server";
   std::lock_guard<std::mutex> lk(s->mu);
   Req* r = find(s, id);
-  return r ? r->error.c_str() : "unknown request";
+  msg = r ? r->error : std::string("unknown request");
+  return msg.c_str();
 }
 
 int atm_server_release(atm_server* s, int64_t id) {

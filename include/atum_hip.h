@@ -320,7 +320,8 @@ int atm_server_destroy(atm_server* s);
 int atm_server_submit(atm_server* s, const atm_request* req, int64_t* id);
 /* the request's state (ATM_REQ_*) or its negative error status */
 int atm_server_poll(atm_server* s, int64_t id);
-/* blocks until the request is done or failed; returns as atm_server_poll */
+/* blocks until the request is done or failed; returns as atm_server_poll
+ * (ATM_REQ_DONE if another thread released it meanwhile) */
 int atm_server_wait(atm_server* s, int64_t id);
 /* the next request that finished or failed and was not yet handed out here:
  * returns 1 and sets *id, or 0 when timeout_s (< 0: no limit) passes first */
@@ -331,7 +332,8 @@ int atm_server_drain(atm_server* s);
 double atm_server_clock(atm_server* s);
 /* submission, admission and completion times of a request (server clock; -1: not yet) */
 int atm_server_times(atm_server* s, int64_t id, double* times3);
-/* message of a failed request ("" otherwise) */
+/* message of a failed request ("" otherwise); a copy owned by the calling
+ * thread, valid until its next call of this function */
 const char* atm_server_error(atm_server* s, int64_t id);
 /* forgets a finished or failed request */
 int atm_server_release(atm_server* s, int64_t id);
