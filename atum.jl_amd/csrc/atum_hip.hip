@@ -2,6 +2,7 @@
 // fused LSRK driver with hipGraph replay, banded vertical solver, ARK23).
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -22,6 +23,11 @@ struct atm_batch {
     std::vector<int> members;
     BatchEntry* dtab = nullptr;
     int64_t blocks = 0;
+    // members of different Nq in one launch: each keeps the shape it had when the
+    // batch was created (threads and lds are the maxima over the members)
+    bool multi = false;
+    std::vector<int> mepw;
+    std::vector<int> mlds;
   };
   std::vector<atm_dg*> dgs;
   std::vector<Group> groups;
@@ -1260,6 +1266,7 @@ int dg_prepare(atm_ctx* ctx, const atm_dg_desc* d, atm_dg** out) {
   dg->lds_fixed = r.lds_fixed;
   dg->lds_elem = r.lds_elem;
   dg->max_threads = r.max_threads;
+  dg->mbfn = r.mbfn;
   dg->mfn = resolve_measures(d->law, d->dim);
   DevDG& p = dg->dev;
   p.epw = r.epw;
@@ -1719,16 +1726,31 @@ int atm_batch_create(atm_ctx* ctx, int32_t n, atm_dg* const* members, atm_batch*
     }
     b->dgs.push_back(dg);
     int g = -1;
-    for (size_t k = 0; k < b->groups.size(); ++k)
-      if (b->groups[k].bfn == dg->bfn && b->groups[k].Np == dg->Np && b->groups[k].epw == dg->epw &&
-          b->groups[k].lds == dg->lds)
+    for (size_t k = 0; k < b->groups.size(); ++k) {
+      const atm_batch::Group& H = b->groups[k];
+      if (dg->mbfn ? (H.multi && H.bfn == dg->mbfn)
+                   : (!H.multi && H.bfn == dg->bfn && H.Np == dg->Np && H.epw == dg->epw && H.lds == dg->lds))
         g = (int)k;
+    }
     if (g < 0) {
       atm_batch::Group G{dg->bfn, dg->Np, dg->epw, ((dg->epw * dg->Np + 63) / 64) * 64, dg->lds, {}, nullptr, 0};
+      if (dg->mbfn) {
+        G.bfn = dg->mbfn;
+        G.multi = true;
+        G.Np = G.epw = G.threads = 0;
+        G.lds = 0;
+      }
       b->groups.push_back(G);
       g = (int)b->groups.size() - 1;
     }
-    b->groups[g].members.push_back(i);
+    atm_batch::Group& G = b->groups[g];
+    G.members.push_back(i);
+    if (G.multi) {
+      G.mepw.push_back(dg->epw);
+      G.mlds.push_back((int)dg->lds);
+      G.threads = std::max(G.threads, ((dg->epw * dg->Np + 63) / 64) * 64);
+      G.lds = std::max(G.lds, dg->lds);
+    }
   }
   hipSetDevice(ctx->device);
   for (auto& G : b->groups) {
@@ -1775,12 +1797,17 @@ int atm_batch_lsrk_run(atm_batch* b, int scheme, double* const* q, double* const
       const int i = G.members[k];
       const atm_dg* dg = b->dgs[i];
       tab[k].p = dg->dev;
+      const int e = G.multi ? G.mepw[k] : G.epw;
+      if (G.multi) {
+        tab[k].p.epw = e;
+        tab[k].p.lds = G.mlds[k];
+      }
       tab[k].buf[0] = q[i];
       tab[k].buf[1] = scratch[i];
       tab[k].dq = dq[i];
       tab[k].dt = dt[i];
       tab[k].block_start = start;
-      start += (dg->dev.K + G.epw - 1) / G.epw;
+      start += (dg->dev.K + e - 1) / e;
     }
     G.blocks = start;
     HIPCHK(ctx, hipMemcpyAsync(G.dtab, tab.data(), tab.size() * sizeof(BatchEntry), hipMemcpyHostToDevice, stream_of(ctx)));

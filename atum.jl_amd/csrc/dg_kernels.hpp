@@ -481,6 +481,43 @@ hipError_t launch_weak_batch(const BatchEntry* tab, int n, int64_t blocks, int t
   return hipGetLastError();
 }
 
+// One launch per stage for weak-form members of different degrees (N = 3..7,
+// Nq = 4..8): each workgroup finds its member in the flattened table and runs
+// that member's Nq-specialised body with the member's own elements per
+// workgroup (p.epw) and LDS share (p.lds <= the launch's LDS); threads beyond
+// a member's epw * Np are inactive, as in every element kernel.
+template <class L, int SF>
+__global__ void __launch_bounds__(512)
+    weak_multi_kernel_batch(const BatchEntry* __restrict__ tab, int n, int parity, double alpha, int assign,
+                            double b) {
+  extern __shared__ double lds[];
+  const int64_t lb = xcd_block();
+  const int i = batch_find(tab, n, lb);
+  const DevDG p = tab[i].p;
+  const int64_t blk = lb - tab[i].block_start;
+  const double* qin = tab[i].buf[parity];
+  double* qo = tab[i].buf[1 - parity];
+  const double bdt = b * tab[i].dt;
+  switch (p.Nq) {
+    case 4: weak_body<L, 4, SF>(p, lds, blk, qin, tab[i].dq, alpha, assign, qo, bdt, p.epw); break;
+    case 5: weak_body<L, 5, SF>(p, lds, blk, qin, tab[i].dq, alpha, assign, qo, bdt, p.epw); break;
+    case 6: weak_body<L, 6, SF>(p, lds, blk, qin, tab[i].dq, alpha, assign, qo, bdt, p.epw); break;
+    case 7: weak_body<L, 7, SF>(p, lds, blk, qin, tab[i].dq, alpha, assign, qo, bdt, p.epw); break;
+    case 8: weak_body<L, 8, SF>(p, lds, blk, qin, tab[i].dq, alpha, assign, qo, bdt, p.epw); break;
+    default: break;
+  }
+}
+
+// `epw` is unused: every member carries its own in the table.
+template <class L, int SF>
+hipError_t launch_weak_multi_batch(const BatchEntry* tab, int n, int64_t blocks, int threads, size_t lds, int epw,
+                                   hipStream_t st, int parity, double alpha, int assign, double b) {
+  (void)epw;
+  hipLaunchKernelGGL((weak_multi_kernel_batch<L, SF>), dim3(blocks), dim3(threads), lds, st, tab, n, parity, alpha,
+                     assign, b);
+  return hipGetLastError();
+}
+
 // ---------------------------------------------------------------------------
 typedef hipError_t (*LaunchFn)(const DevDG& p, hipStream_t st, const double* q, double* dq,
                                double alpha, int assign, double* qout, double bdt);
@@ -536,6 +573,9 @@ struct Resolved {
   int epw;     // default elements per workgroup
   size_t lds_fixed, lds_elem;
   int max_threads = 512;  // largest workgroup the kernels are built for
+  // batched launcher shared by the members of one family of different Nq (one
+  // launch per stage for all of them), or null
+  BatchLaunchFn mbfn = nullptr;
 };
 
 inline Resolved make_resolved(LaunchFn fn, BatchLaunchFn bfn, int ns, int epw, size_t fixed, size_t elem,

@@ -34,6 +34,7 @@ struct atm_dg {
   int epw = 1;
   size_t lds_fixed = 0, lds_elem = 0;  // LDS bytes of a workgroup = lds_fixed + epw * lds_elem
   int max_threads = 512;               // largest workgroup of the element kernels
+  BatchLaunchFn mbfn = nullptr;         // one-launch batched kernel across Nq (or null)
   int32_t* sched = nullptr;             // device copy of the element processing order (or null)
   int law = 0, dim = 0, Nq = 0, Np = 0, Ns = 0;
   int64_t Ntot = 0;

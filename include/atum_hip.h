@@ -239,8 +239,11 @@ int atm_ark23_run(atm_dg* ex, atm_dg* lin, atm_banded* band, double* q, double* 
  * share a kernel (law, dim, N, forms, fluxes, precision) form a group whose
  * elements are laid out back to back in a flattened element table (per-member
  * first workgroup), and each group advances in ONE launch per stage; with
- * use_graph the two-step stage sequence of all groups is one hipGraph.  No
- * counterpart in the reference (it runs one DGSEM per solver). */
+ * use_graph the two-step stage sequence of all groups is one hipGraph.  Kernel
+ * families with a multi-degree kernel (weak-form shallow water with Rusanov
+ * surface flux, N = 3..7) put all their degrees in one group.  Each member's
+ * elements per workgroup are taken when the batch is created.  No counterpart
+ * in the reference (it runs one DGSEM per solver). */
 typedef struct atm_batch atm_batch;
 int atm_batch_create(atm_ctx* ctx, int32_t n, atm_dg* const* members, atm_batch** out);
 int atm_batch_destroy(atm_batch* b);
