@@ -1752,6 +1752,21 @@ int atm_batch_create(atm_ctx* ctx, int32_t n, atm_dg* const* members, atm_batch*
       G.lds = std::max(G.lds, dg->lds);
     }
   }
+  // a multi-degree group launches workgroups of its largest member's size: the
+  // other members take as many elements per workgroup as fill it (results do not
+  // depend on the shape) while their LDS fits
+  for (auto& G : b->groups) {
+    if (!G.multi) continue;
+    G.lds = 0;
+    for (size_t k = 0; k < G.members.size(); ++k) {
+      const atm_dg* dg = b->dgs[G.members[k]];
+      int e = std::max(G.mepw[k], G.threads / dg->Np);
+      while (e > G.mepw[k] && dg->lds_fixed + (size_t)e * dg->lds_elem > ctx->max_lds) --e;
+      G.mepw[k] = e;
+      G.mlds[k] = (int)(dg->lds_fixed + (size_t)e * dg->lds_elem);
+      G.lds = std::max(G.lds, (size_t)G.mlds[k]);
+    }
+  }
   hipSetDevice(ctx->device);
   for (auto& G : b->groups) {
     hipError_t e = hipMalloc(&G.dtab, G.members.size() * sizeof(BatchEntry));

@@ -323,6 +323,42 @@ fd_fast_kernel_batch(const BatchEntry* __restrict__ tab, int n, int parity, doub
                           tab[i].buf[1 - parity], b * tab[i].dt, epw);
 }
 
+// One launch per stage for 2-D members of different degrees (N = 3..7, Nq =
+// 4..8) with the same law and surface flux: each workgroup runs its member's
+// Nq-specialised body with the member's own elements per workgroup and LDS
+// share from the table (as weak_multi_kernel_batch).
+template <class L, int SF>
+__global__ void __launch_bounds__(512)
+    fd_fast_multi_kernel_batch(const BatchEntry* __restrict__ tab, int n, int parity, double alpha, int assign,
+                               double b) {
+  extern __shared__ double lds[];
+  const int64_t lb = xcd_block();
+  const int i = batch_find(tab, n, lb);
+  const DevDG p = tab[i].p;
+  const int64_t blk = lb - tab[i].block_start;
+  const double* qin = tab[i].buf[parity];
+  double* qo = tab[i].buf[1 - parity];
+  const double bdt = b * tab[i].dt;
+  switch (p.Nq) {
+    case 4: fd_fast_body<L, 4, SF>(p, lds, blk, qin, tab[i].dq, alpha, assign, qo, bdt, p.epw); break;
+    case 5: fd_fast_body<L, 5, SF>(p, lds, blk, qin, tab[i].dq, alpha, assign, qo, bdt, p.epw); break;
+    case 6: fd_fast_body<L, 6, SF>(p, lds, blk, qin, tab[i].dq, alpha, assign, qo, bdt, p.epw); break;
+    case 7: fd_fast_body<L, 7, SF>(p, lds, blk, qin, tab[i].dq, alpha, assign, qo, bdt, p.epw); break;
+    case 8: fd_fast_body<L, 8, SF>(p, lds, blk, qin, tab[i].dq, alpha, assign, qo, bdt, p.epw); break;
+    default: break;
+  }
+}
+
+// `epw` is unused: every member carries its own in the table.
+template <class L, int SF>
+hipError_t launch_fd_fast_multi_batch(const BatchEntry* tab, int n, int64_t blocks, int threads, size_t lds, int epw,
+                                      hipStream_t st, int parity, double alpha, int assign, double b) {
+  (void)epw;
+  hipLaunchKernelGGL((fd_fast_multi_kernel_batch<L, SF>), dim3(blocks), dim3(threads), lds, st, tab, n, parity,
+                     alpha, assign, b);
+  return hipGetLastError();
+}
+
 template <class L, int NQ, int SF>
 constexpr int fast_epw() {
   constexpr int Np = L::D == 1 ? NQ : (L::D == 2 ? NQ * NQ : NQ * NQ * NQ);

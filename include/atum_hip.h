@@ -241,9 +241,12 @@ int atm_ark23_run(atm_dg* ex, atm_dg* lin, atm_banded* band, double* q, double* 
  * first workgroup), and each group advances in ONE launch per stage; with
  * use_graph the two-step stage sequence of all groups is one hipGraph.  Kernel
  * families with a multi-degree kernel (weak-form shallow water with Rusanov
- * surface flux, N = 3..7) put all their degrees in one group.  Each member's
- * elements per workgroup are taken when the batch is created.  No counterpart
- * in the reference (it runs one DGSEM per solver). */
+ * surface flux; the 2-D FAST Euler and Euler-gravity kernels with Roe or
+ * Rusanov surface flux; N = 3..7) put all their degrees in one group, whose
+ * workgroups have the size of its largest member's (smaller members take more
+ * elements per workgroup).  Each member's elements per workgroup are taken when
+ * the batch is created.  No counterpart in the reference (it runs one DGSEM per
+ * solver). */
 typedef struct atm_batch atm_batch;
 int atm_batch_create(atm_ctx* ctx, int32_t n, atm_dg* const* members, atm_batch** out);
 int atm_batch_destroy(atm_batch* b);

@@ -34,6 +34,32 @@ def test_batch_equals_alone_shallow_water(nsteps):
         assert np.array_equal(q.numpy(), alone.numpy())
 
 
+@pytest.mark.parametrize("sflux", ["rusanov", "roe"])
+def test_batch_fast_euler_all_degrees_one_launch(sflux):
+    """2-D Euler vortices of N = 3..7 (FAST EC volume) advance in ONE launch per stage;
+    members whose own shape is smaller than the launch's workgroup take more
+    elements per workgroup; each ends bitwise where it would alone."""
+    import atum_hip as A
+    members = []
+    for N, K in ((3, 5), (4, 4), (5, 3), (6, 3), (7, 2)):
+        v = P.julia_range(-5, 5, K + 1)
+        g = A.brickgrid(A.LobattoCell(N, 2), (v, v), (True, True))
+        flux = A.RusanovFlux() if sflux == "rusanov" else A.RoeFlux()
+        dg = A.DGSEM(law=A.EulerLaw(2), grid=g, volume_form=A.FluxDifferencingForm(A.EntropyConservativeFlux()),
+                     surface_numericalflux=flux, precision="fast")
+        members.append((dg, P.isentropic_vortex(g.points, 0.0, center=(0.0, 0.0), u0=(1.0, 1.0)), 0.02))
+    dgs = [m[0] for m in members]
+    qs = [dg.fieldarray(m[1]) for dg, m in zip(dgs, members)]
+    ens = A.LSRKEnsemble(dgs, qs, [m[2] for m in members])
+    assert ens.batch.ngroups == 1
+    ens.run(4)
+    for (dg, q0, dt), q in zip(members, qs):
+        alone = dg.fieldarray(q0)
+        A.LSRK54(dg, alone, dt).run(alone, 4)
+        dg.ctx.sync()
+        assert np.array_equal(q.numpy(), alone.numpy())
+
+
 def test_batch_mixed_laws_and_modes():
     """Vortex (EC, fast and reference), bubble (EC + Roe) and shallow water in one batch."""
     import atum_hip as A
