@@ -435,8 +435,8 @@ def main():
                     help="side streams for the IMEX requests of the served stream (0: the main stream)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py: this build measures one GPU; the path shards by elements with one face "
-              "exchange per stage (see DESIGN.md), but no multi-GPU run is made here.", file=sys.stderr)
+        print("bench.py: measures one GPU only; the path would shard by elements with one face "
+              "exchange per stage, but no multi-GPU run is made here.", file=sys.stderr)
         sys.exit(2)
     if args.steps % 2:
         args.steps += 1  # whole two-step graphs: the state returns to q every pair

@@ -4,7 +4,7 @@
  * Plain C: opaque handles, POD descriptors, double / int64 / int32 pointers,
  * int status (0 = ok, < 0 = error; details from atm_last_error).  No C++
  * exceptions and no torch types cross this boundary, so Julia can `ccall` it
- * directly (the binding is shown in INTEGRATION.md).
+ * directly.
  *
  * Data layout (the reference's data model, Atum/Bennu field arrays):
  *   - a field with S components is S contiguous blocks of Np*K doubles; block s
