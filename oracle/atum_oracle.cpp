@@ -1,7 +1,7 @@
 // CPU ORACLE — test infrastructure only.
 //
 // FP64 CPU restatement of Atum.jl's DGSEM tendency and time-step path.  It is
-// the checker for the HIP library: only the tests, the smoke check and the
+// the reference the HIP library is tested against: only the tests, the smoke check and the
 // bench's CPU-baseline leg load it.  The product library never links it.
 //
 // Restated from (paths relative to the reference checkout):

@@ -3,7 +3,7 @@
 ctypes front end of ``oracle/build/liboracle.so``, the FP64 CPU restatement of
 Atum.jl's DGSEM tendency and time-step path (see ``atum_oracle.cpp`` for the
 reference file:line each part follows).  Only the tests, the smoke check and
-the bench's CPU-baseline leg import this module; it is the checker, never the
+the bench's CPU-baseline leg import this module; it is what results are compared with, never the
 thing measured or shipped.
 
 Arrays follow the reference data model: one FP64 array per state component,
