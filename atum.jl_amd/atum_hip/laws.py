@@ -114,7 +114,7 @@ class AdvectionLaw(AbstractBalanceLaw):
 
 
 class EulerLaw(AbstractBalanceLaw):
-    """src/balancelaws/euler.jl:29-37."""
+    """src/balancelaws/euler.jl:9-17."""
     law = "euler"
 
     def __init__(self, dim, gamma=1.4, problem=None):

@@ -22,6 +22,11 @@ struct atm_batch {
     size_t lds;
     std::vector<int> members;
     BatchEntry* dtab = nullptr;
+    // pinned host copy of the table the device holds (the source of its uploads,
+    // kept for the batch's life) and an event after the last upload from it
+    BatchEntry* htab = nullptr;
+    hipEvent_t uploaded = nullptr;
+    bool valid = false;  // htab/dtab hold a table
     int64_t blocks = 0;
     // members of different Nq in one launch: each keeps the shape it had when the
     // batch was created (threads and lds are the maxima over the members)
@@ -1733,7 +1738,12 @@ int atm_batch_create(atm_ctx* ctx, int32_t n, atm_dg* const* members, atm_batch*
         g = (int)k;
     }
     if (g < 0) {
-      atm_batch::Group G{dg->bfn, dg->Np, dg->epw, ((dg->epw * dg->Np + 63) / 64) * 64, dg->lds, {}, nullptr, 0};
+      atm_batch::Group G;
+      G.bfn = dg->bfn;
+      G.Np = dg->Np;
+      G.epw = dg->epw;
+      G.threads = ((dg->epw * dg->Np + 63) / 64) * 64;
+      G.lds = dg->lds;
       if (dg->mbfn) {
         G.bfn = dg->mbfn;
         G.multi = true;
@@ -1769,10 +1779,13 @@ int atm_batch_create(atm_ctx* ctx, int32_t n, atm_dg* const* members, atm_batch*
   }
   hipSetDevice(ctx->device);
   for (auto& G : b->groups) {
-    hipError_t e = hipMalloc(&G.dtab, G.members.size() * sizeof(BatchEntry));
+    const size_t bytes = G.members.size() * sizeof(BatchEntry);
+    hipError_t e = hipMalloc(&G.dtab, bytes);
+    if (e == hipSuccess) e = hipHostMalloc((void**)&G.htab, bytes, hipHostMallocDefault);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&G.uploaded, hipEventDisableTiming);
     if (e != hipSuccess) {
       atm_batch_destroy(b);
-      return hipfail(ctx, e, "hipMalloc batch table");
+      return hipfail(ctx, e, "batch table allocation");
     }
   }
   *out = b;
@@ -1785,8 +1798,11 @@ int atm_batch_destroy(atm_batch* b) {
   hipStreamSynchronize(stream_of(b->ctx));
   for (hipGraphExec_t& x : b->exec)
     if (x) hipGraphExecDestroy(x);
-  for (auto& G : b->groups)
+  for (auto& G : b->groups) {
     if (G.dtab) hipFree(G.dtab);
+    if (G.htab) hipHostFree(G.htab);
+    if (G.uploaded) hipEventDestroy(G.uploaded);
+  }
   delete b;
   return ATM_OK;
 }
@@ -1804,9 +1820,13 @@ int atm_batch_lsrk_run(atm_batch* b, int scheme, double* const* q, double* const
     if (!q[i] || !dq[i] || !scratch[i] || q[i] == scratch[i] || q[i] == dq[i] || dq[i] == scratch[i])
       return fail(ctx, ATM_ERR_ARG, "atm_batch_lsrk_run: member buffers missing or aliased");
   }
-  // device tables (flattened element table: first workgroup of each member)
+  // device tables (flattened element table: first workgroup of each member).  The
+  // table is uploaded from the group's pinned host copy, and only when it changed
+  // (members' buffers, time steps or shapes); before the host copy is rewritten,
+  // the previous upload from it must have finished.
   for (auto& G : b->groups) {
     std::vector<BatchEntry> tab(G.members.size());
+    std::memset(tab.data(), 0, tab.size() * sizeof(BatchEntry));  // padding too: compared bytewise
     int64_t start = 0;
     for (size_t k = 0; k < G.members.size(); ++k) {
       const int i = G.members[k];
@@ -1825,7 +1845,13 @@ int atm_batch_lsrk_run(atm_batch* b, int scheme, double* const* q, double* const
       start += (dg->dev.K + e - 1) / e;
     }
     G.blocks = start;
-    HIPCHK(ctx, hipMemcpyAsync(G.dtab, tab.data(), tab.size() * sizeof(BatchEntry), hipMemcpyHostToDevice, stream_of(ctx)));
+    const size_t bytes = tab.size() * sizeof(BatchEntry);
+    if (G.valid && std::memcmp(G.htab, tab.data(), bytes) == 0) continue;
+    HIPCHK(ctx, hipEventSynchronize(G.uploaded));
+    std::memcpy(G.htab, tab.data(), bytes);
+    HIPCHK(ctx, hipMemcpyAsync(G.dtab, G.htab, bytes, hipMemcpyHostToDevice, stream_of(ctx)));
+    HIPCHK(ctx, hipEventRecord(G.uploaded, stream_of(ctx)));
+    G.valid = true;
   }
   const LSRKTable t = lsrk_table(scheme);
   int parity = 0;

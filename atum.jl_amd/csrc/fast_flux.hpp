@@ -1,5 +1,5 @@
 // Surface fluxes of the FAST kernels for the Euler laws: the Roe flux
-// (src/balancelaws/euler.jl:111-158, euler_gravity.jl:149-198: the minus-side
+// (src/balancelaws/euler.jl:91-138, euler_gravity.jl:149-198: the minus-side
 // geopotential in both pressures of the dissipation, each side's own aux in
 // its physical flux) and the Rusanov flux (src/numericalfluxes.jl:36-42).
 //

@@ -9,7 +9,7 @@
 //   avg / logavg / roe_avg ........ src/numericalfluxes.jl:8-23
 //   Central / Rusanov / default ... src/numericalfluxes.jl:26-42
 //   AdvectionLaw .................. src/balancelaws/advection.jl:8-19
-//   EulerLaw ...................... src/balancelaws/euler.jl:29-256
+//   EulerLaw ...................... src/balancelaws/euler.jl:9-256
 //   EulerGravityLaw ............... src/balancelaws/euler_gravity.jl:10-332
 //   LinearEulerGravityLaw ......... src/balancelaws/linear_euler_gravity.jl:3-65
 //   ShallowWaterLaw ............... src/balancelaws/shallow_water.jl:9-136
@@ -390,7 +390,7 @@ ATM_DEV void LinearEulerGravity<D>::twopoint(const LawConst& c, const double* q1
 
 // Ismail–Roe / Chandrashekar entropy-conservative and Kennedy–Gruber fluxes for
 // the Euler laws; with gravity the fluctuation term -α(Φ1-Φ2)I is added
-// (euler.jl:160-216, euler_gravity.jl:200-269).
+// (euler.jl:140-196, euler_gravity.jl:200-269).
 template <int D, bool GRAV, int VF>
 ATM_DEV void euler_twopoint(const LawConst& c, const double* q1, const double* a1,
                             const double* q2, const double* a2, double F[D][D + 2]) {
@@ -514,7 +514,7 @@ ATM_DEV void ShallowWater<D>::twopoint(const LawConst& c, const double* q1, cons
   }
 }
 
-// Roe flux for the Euler laws (euler.jl:111-158; euler_gravity.jl:149-198, which
+// Roe flux for the Euler laws (euler.jl:91-138; euler_gravity.jl:149-198, which
 // uses the minus-side geopotential for both pressures).
 template <int D, bool GRAV>
 ATM_DEV void euler_roe(const LawConst& c, const double* n, const double* qm, const double* am,
@@ -589,7 +589,7 @@ ATM_DEV void euler_roe(const LawConst& c, const double* n, const double* qm, con
   }
 }
 
-// Matrix (entropy-stable) flux (euler.jl:218-276; euler_gravity.jl:271-332)
+// Matrix (entropy-stable) flux (euler.jl:198-256; euler_gravity.jl:271-332)
 template <int D, bool GRAV>
 ATM_DEV void euler_matrix(const LawConst& c, const double* n, const double* qm, const double* am,
                           const double* qp, const double* ap, double* nf) {

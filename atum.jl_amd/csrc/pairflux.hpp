@@ -6,7 +6,7 @@
 // only the genuinely two-point arithmetic.  Each primitive is computed with
 // exactly the operations the reference's flux evaluates for it, so the pair
 // result is bit-identical to evaluating the flux from the two states
-// (src/balancelaws/euler.jl:160-216, euler_gravity.jl:200-269,
+// (src/balancelaws/euler.jl:140-196, euler_gravity.jl:200-269,
 // shallow_water.jl:112-136, src/numericalfluxes.jl:31-34).
 #pragma once
 #include "laws.hpp"

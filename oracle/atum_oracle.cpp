@@ -467,6 +467,39 @@ struct OpEntropyProduct {
   static double fail() { return NAN; }
 };
 
+// Point evaluations of one law's fluxes (for the flux-level checks).
+template <class L>
+struct OpPointFlux {
+  // which: 0 physical flux F(q1) (D x Ns, row d), 1 two-point flux `flux` of
+  // (q1, a1; q2, a2) (D x Ns), 2 surface flux `flux` with normal n of
+  // (q1, a1; q2, a2) (Ns), 3 entropy variables of q1 (Ns)
+  static int run(const Ctx& c, int which, int flux, const double* n, const double* q1, const double* a1,
+                 const double* q2, const double* a2, double* out) {
+    constexpr int D = L::D, Ns = L::Ns;
+    try {
+      if (which == 0 || which == 1) {
+        double F[D][Ns];
+        if (which == 0)
+          L::flux(c.P, q1, a1, F);
+        else
+          L::twopoint(flux, c.P, q1, a1, q2, a2, F);
+        for (int d = 0; d < D; ++d)
+          for (int s = 0; s < Ns; ++s) out[d * Ns + s] = F[d][s];
+      } else if (which == 2) {
+        surfaceflux<L>(flux, c.P, n, q1, a1, q2, a2, out);
+      } else if (which == 3) {
+        L::entropyvariables(c.P, q1, a1, out);
+      } else {
+        return -2;
+      }
+    } catch (Unsupported&) {
+      return -3;
+    }
+    return 0;
+  }
+  static int fail() { return -1; }
+};
+
 int nstates(const Ctx& c) { return dispatch<OpNs>(c); }
 
 int check(const orc_dg* o) {
@@ -899,6 +932,14 @@ double orc_weightednorm(int Ns, int64_t Ntot, const double* MJ, const double* q)
     tot += (long double)ns * ns;
   }
   return std::sqrt((double)tot);
+}
+
+// One flux evaluated at one point pair (law, dimension and constants from o);
+// see OpPointFlux for `which`.  Returns 0, or < 0 for an unsupported law x flux.
+int orc_pointflux(const orc_dg* o, int which, int flux, const double* n, const double* q1, const double* a1,
+                  const double* q2, const double* a2, double* out) {
+  Ctx c = make_ctx(o);
+  return dispatch<OpPointFlux>(c, which, flux, n, q1, a1, q2, a2, out);
 }
 
 double orc_entropyintegral(const orc_dg* o, const double* q) {

@@ -10,7 +10,7 @@
 //
 //   src/numericalfluxes.jl:8-47            avg, logavg, roe_avg, Central, Rusanov
 //   src/balancelaws/advection.jl:8-19      AdvectionLaw
-//   src/balancelaws/euler.jl:29-256        EulerLaw (flux, Roe, EC, KG, Matrix)
+//   src/balancelaws/euler.jl:9-256         EulerLaw (flux, Roe, EC, KG, Matrix)
 //   src/balancelaws/euler_gravity.jl:10-332 EulerGravityLaw
 //   src/balancelaws/linear_euler_gravity.jl:3-65 LinearEulerGravityLaw
 //   src/balancelaws/shallow_water.jl:9-136 ShallowWaterLaw
@@ -103,14 +103,14 @@ struct Advection {
 template <int D_>
 struct Euler {
   static constexpr int D = D_, Ns = D_ + 2;
-  // :52-55
+  // :32-43 (pressure, soundspeed)
   static double pressure(const Params& P, double rho, const double* rhou, double rhoe) {
     return (P.gamma - 1) * (rhoe - dotv<D>(rhou, rhou) / (2 * rho));
   }
   static double soundspeed(const Params& P, double rho, double p) {
     return std::sqrt(P.gamma * p / rho);
   }
-  // :68-79
+  // :48-59
   static void flux(const Params& P, const double* q, const double*, double F[D][Ns]) {
     double rho = q[0], rhoe = q[D + 1];
     const double* rhou = q + 1;
@@ -124,21 +124,21 @@ struct Euler {
       F[d][D + 1] = u[d] * (rhoe + p);
     }
   }
-  // :81-86
+  // :61-66
   static double wavespeed(const Params& P, const double* n, const double* q, const double*) {
     double rho = q[0];
     double u[D];
     for (int d = 0; d < D; ++d) u[d] = q[1 + d] / rho;
     return std::fabs(dotv<D>(n, u)) + soundspeed(P, rho, pressure(P, rho, q + 1, q[D + 1]));
   }
-  // :88-95
+  // :68-75
   static double entropy(const Params& P, const double* q, const double*) {
     double rho = q[0];
     double p = pressure(P, rho, q + 1, q[D + 1]);
     double s = std::log(p / std::pow(rho, P.gamma));
     return -rho * s / (P.gamma - 1);
   }
-  // :97-109
+  // :77-89
   static void entropyvariables(const Params& P, const double* q, const double*, double* v) {
     double rho = q[0];
     double g = P.gamma;
@@ -381,7 +381,7 @@ void LinearEulerGravity<D>::twopoint(int vf, const Params& P, const double* q1, 
   throw Unsupported();
 }
 
-// Euler EC: src/balancelaws/euler.jl:160-189;  KG: :191-216
+// Euler EC: src/balancelaws/euler.jl:140-169;  KG: :171-196
 template <int D>
 void Euler<D>::twopoint(int vf, const Params& P, const double* q1, const double* a1,
                         const double* q2, const double* a2, double F[D][Ns]) {
@@ -535,7 +535,7 @@ void ShallowWater<D>::twopoint(int vf, const Params& P, const double* q1, const 
 // ---------------------------------------------------------------------------
 // Roe / Matrix surface fluxes
 
-// Euler Roe: src/balancelaws/euler.jl:111-158 ; Matrix: :218-276
+// Euler Roe: src/balancelaws/euler.jl:91-138 ; Matrix: :198-256
 template <int D>
 void Euler<D>::surface_special(int sf, const Params& P, const double* n, const double* qm,
                                const double* am, const double* qp, const double* ap, double* nf) {

@@ -18,7 +18,9 @@ import subprocess
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-_SO = os.path.join(_HERE, "build", "liboracle.so")
+# ATUM_ORACLE_SO names another build of the same sources (the mutation checks
+# of the flux tests load deliberately broken copies this way)
+_SO = os.environ.get("ATUM_ORACLE_SO") or os.path.join(_HERE, "build", "liboracle.so")
 
 LAWS = {"advection": 0, "euler": 1, "euler_gravity": 2,
         "linear_euler_gravity": 3, "shallow_water": 4}
@@ -301,3 +303,35 @@ def solve_ark23(ex, lin, q, dt, timeend, *, split_rhs=True, paperversion=False, 
         ctypes.c_int(ncols), ctypes.c_int64(max_steps), ctypes.byref(n), ctypes.byref(t))
     assert rc == 0
     return n.value, t.value
+
+
+_POINT_WHICH = {"flux": 0, "twopoint": 1, "surface": 2, "entropyvariables": 3}
+
+
+def point_flux(law, dim, which, q1, a1=None, q2=None, a2=None, *, flux="central", n=None, gamma=1.4,
+               grav=9.81):
+    """One flux of ``law`` at one point (pair), as the oracle's tendency evaluates it.
+
+    which: "flux" F(q1) -> (dim, Ns); "twopoint" -> (dim, Ns); "surface" (normal n)
+    -> (Ns,); "entropyvariables" of q1 -> (Ns,).
+    """
+    s = _DG()
+    s.geom.dim, s.geom.Nq, s.geom.K = dim, 2, 0
+    q1 = np.ascontiguousarray(q1, dtype=np.float64)
+    Ns = q1.shape[0]
+    a1 = np.ascontiguousarray(np.zeros(1) if a1 is None else a1, dtype=np.float64)
+    q2 = np.ascontiguousarray(q1 if q2 is None else q2, dtype=np.float64)
+    a2 = np.ascontiguousarray(a1 if a2 is None else a2, dtype=np.float64)
+    n = np.ascontiguousarray(np.zeros(dim) if n is None else n, dtype=np.float64)
+    s.geom.naux = a1.shape[0]
+    s.law = LAWS[law]
+    s.gamma, s.grav = float(gamma), float(grav)
+    w = _POINT_WHICH[which]
+    out = np.zeros(dim * Ns if w in (0, 1) else Ns)
+    L = lib()
+    L.orc_pointflux.restype = ctypes.c_int
+    rc = L.orc_pointflux(ctypes.byref(s), ctypes.c_int(w), ctypes.c_int(FLUXES[flux]), _p(n), _p(q1), _p(a1),
+                         _p(q2), _p(a2), _p(out))
+    if rc != 0:
+        raise ValueError(f"oracle: {flux} {which} unsupported for {law} (code {rc})")
+    return out.reshape(dim, Ns) if w in (0, 1) else out

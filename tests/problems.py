@@ -28,7 +28,7 @@ def julia_step(a, b, length):
 
 
 def euler_energy(gamma, rho, rhou, p):
-    """src/balancelaws/euler.jl:56-59: p/(γ-1) + ρu⃗'ρu⃗/2ρ."""
+    """src/balancelaws/euler.jl:36-39: p/(γ-1) + ρu⃗'ρu⃗/2ρ."""
     ke = rhou[0] * rhou[0]
     for d in range(1, len(rhou)):
         ke = ke + rhou[d] * rhou[d]
