@@ -33,6 +33,7 @@ FLUX_IDS = {"central": 0, "rusanov": 1, "ec": 2, "kg": 3, "roe": 4, "matrix": 5}
 BC_IDS = {"copy": 0, "reflect": 1, "zero_inflow": 2, "prescribed": 3}
 SOURCE_IDS = {None: 0, "coriolis": 1}
 PRECISION_IDS = {"reference": 0, "fast": 1}
+FD_KERNEL_IDS = {"per_dir": 0, "per_dir_symmetric": 1, "naive": 2}
 
 
 class DgDesc(ctypes.Structure):
@@ -40,7 +41,7 @@ class DgDesc(ctypes.Structure):
                 ("volume_form", c_int32), ("volume_flux", c_int32), ("surface_flux", c_int32),
                 ("dirmask", c_int32), ("bc", c_int32), ("source", c_int32),
                 ("pde_level_balance", c_int32), ("naux", c_int32), ("n_bc_states", c_int32),
-                ("precision", c_int32), ("gamma", c_double), ("grav", c_double),
+                ("precision", c_int32), ("fd_kernel", c_int32), ("reserved", c_int32), ("gamma", c_double), ("grav", c_double),
                 ("omega", c_double), ("u", c_double * 3), ("bc_states", _dp), ("D", _dp),
                 ("g", _dp), ("MJ", _dp), ("MJI", _dp), ("faceMJ", _dp), ("normal", _dp),
                 ("aux", _dp), ("faceix_m", _i64p), ("faceix_p", _i64p), ("bctag", _i32p)]
@@ -88,6 +89,7 @@ _sig("atm_dg_geometry", c_int, [c_void_p, c_int, c_void_p])
 _sig("atm_dg_destroy", c_int, [c_void_p])
 _sig("atm_dg_nstates", c_int, [c_void_p])
 _sig("atm_dg_nnodes", c_int64, [c_void_p])
+_sig("atm_dg_device_bytes", c_int64, [c_void_p])
 _sig("atm_dg_get_epw", c_int, [c_void_p])
 _sig("atm_dg_set_schedule", c_int, [c_void_p, ctypes.POINTER(ctypes.c_int32), c_int64])
 _sig("atm_dg_set_epw", c_int, [c_void_p, c_int])
@@ -105,6 +107,7 @@ _sig("atm_banded_solve", c_int, [c_void_p, c_void_p, c_void_p])
 _sig("atm_banded_shape", c_int, [c_void_p, _i64p, _i32p, _i32p])
 _sig("atm_banded_download", c_int, [c_void_p, _dp])
 _sig("atm_banded_nunique", c_int64, [c_void_p])
+_sig("atm_banded_device_bytes", c_int64, [c_void_p])
 _sig("atm_ark23_run", c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_double, c_double,
                               c_int64, c_int, c_int])
 _sig("atm_event_record", c_int, [c_void_p, c_int])

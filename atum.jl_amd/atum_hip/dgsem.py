@@ -30,9 +30,12 @@ class WeakForm:
 
 
 class FluxDifferencingForm:
-    """src/dgsem.jl:8-14.  All three reference kernel variants (:per_dir,
-    :per_dir_symmetric, :naive) compute the same sum; the device kernel keeps
-    the :per_dir order."""
+    """src/dgsem.jl:8-14.  The three reference kernel variants (:per_dir,
+    :per_dir_symmetric, :naive; src/dgsem.jl:135-212) compute the same sum in
+    different orders; in precision="reference" the device kernel follows the
+    order `kernel_type` names (their last bits differ), and "naive" needs all
+    directions, as in the reference.  precision="fast" evaluates its own
+    pair-symmetric order and accepts only the default kernel_type."""
 
     def __init__(self, volume_numericalflux: AbstractNumericalFlux = EntropyConservativeFlux(),
                  kernel_type: str = "per_dir"):
@@ -42,14 +45,15 @@ class FluxDifferencingForm:
         self.kernel_type = kernel_type
 
     def __repr__(self):
-        return f"FluxDifferencingForm({self.volume_numericalflux!r})"
+        return f"FluxDifferencingForm({self.volume_numericalflux!r}, {self.kernel_type!r})"
 
     def __eq__(self, other):
         return (isinstance(other, FluxDifferencingForm)
-                and other.volume_numericalflux == self.volume_numericalflux)
+                and other.volume_numericalflux == self.volume_numericalflux
+                and other.kernel_type == self.kernel_type)
 
     def __hash__(self):
-        return hash(("FD", self.volume_numericalflux))
+        return hash(("FD", self.volume_numericalflux, self.kernel_type))
 
 
 class DGSEM:
@@ -95,6 +99,7 @@ class DGSEM:
         else:
             d.volume_form = 1
             d.volume_flux = _lib.FLUX_IDS[self.volume_form.volume_numericalflux.name]
+            d.fd_kernel = _lib.FD_KERNEL_IDS[self.volume_form.kernel_type]
         d.surface_flux = _lib.FLUX_IDS[surface_numericalflux.name]
         d.dirmask = sum(1 << (r - 1) for r in self.directions)
         d.bc = _lib.BC_IDS[prob.boundary]
@@ -105,7 +110,7 @@ class DGSEM:
         self.precision = precision
         # what selects the element kernel (with K, the key of the workgroup-shape tuning)
         self.kernel_key = (law.law, dim, grid.N, d.volume_form, d.volume_flux, d.surface_flux, d.dirmask,
-                           precision)
+                           precision, d.fd_kernel)
         d.gamma = float(getattr(law, "gamma", 1.4))
         d.grav = float(getattr(law, "grav", 9.81))
         d.omega = float(prob.omega)

@@ -45,10 +45,13 @@ class _Req(ctypes.Structure):
 
 _STAT_INT = ("submitted", "completed", "failed", "ticks", "max_active")
 _STAT_DBL = ("dof_updates", "busy_s", "gpu_span_s", "admit_s", "enqueue_s", "wait_s", "retire_s")
+_STAT_POOL_INT = ("pool_hits", "pool_misses")
+_STAT_MEM = ("peak_bytes", "peak_operator_bytes", "pool_bytes")
 
 
 class _Stats(ctypes.Structure):
-    _fields_ = [(n, c_int64) for n in _STAT_INT] + [(n, c_double) for n in _STAT_DBL]
+    _fields_ = ([(n, c_int64) for n in _STAT_INT] + [(n, c_double) for n in _STAT_DBL]
+                + [(n, c_int64) for n in _STAT_POOL_INT] + [(n, c_double) for n in _STAT_MEM])
 
 
 _P = ctypes.POINTER
@@ -101,6 +104,7 @@ class ServeStats:
     wait_s: float = 0.0      # worker: waiting for the device
     retire_s: float = 0.0    # worker: downloading results, freeing
     gpu_span_s: float = 0.0  # device: from the first to the last kernel of each tick (events)
+    clock0: float = 0.0      # scheduler clock at the start of the run (arrival times count from it)
 
     def summary(self):
         lat = np.array(self.latencies) if self.latencies else np.zeros(1)
@@ -119,6 +123,7 @@ class Server:
         it, timed once per kernel configuration and size class on first use (while the
         device serves other requests, so the timing is taken under load)."""
         self.ctx = ctx or _lib.context()
+        self.hbm_budget = float(hbm_budget)
         self.tune = tune
         self._tuned = set()
         self.keep_results = keep_results
@@ -198,7 +203,15 @@ class Server:
     def stats(self) -> dict:
         st = _Stats()
         _lib.lib.atm_server_get_stats(self.h, ctypes.byref(st))
-        return {n: getattr(st, n) for n in _STAT_INT + _STAT_DBL}
+        return {n: getattr(st, n) for n in _STAT_INT + _STAT_DBL + _STAT_POOL_INT + _STAT_MEM}
+
+    def footprint_stats(self) -> dict:
+        """Device memory as the scheduler counted it against its budget: the peak of
+        resident buffers plus operators, the operators' share, and the idle pool."""
+        st = self.stats()
+        return {"budget_bytes": self.hbm_budget, "peak_bytes": st["peak_bytes"],
+                "peak_operator_bytes": st["peak_operator_bytes"], "pool_bytes": st["pool_bytes"],
+                "pool_hits": st["pool_hits"], "pool_misses": st["pool_misses"]}
 
     def run(self, requests):
         """Serve `requests`: a request is submitted when the wall clock since the start of
@@ -209,6 +222,7 @@ class Server:
         stats = ServeStats()
         t0 = time.perf_counter()
         c0 = self.clock()
+        stats.clock0 = c0
         pi, waiting, failed = 0, 0, []
         while pi < len(pending) or waiting:
             tb = time.perf_counter()

@@ -16,17 +16,23 @@ using namespace atm;
 
 struct atm_batch {
   atm_ctx* ctx = nullptr;
+  // a device table and its pinned host copy (the source of its uploads, kept for
+  // the batch's life), with an event after the last upload from it; a slot keeps
+  // its memory when the members change and grows only when a group outgrows it
+  struct Slot {
+    BatchEntry* dtab = nullptr;
+    BatchEntry* htab = nullptr;
+    hipEvent_t uploaded = nullptr;
+    size_t cap = 0;      // entries
+    size_t used = 0;     // entries of the table it holds
+    bool valid = false;  // htab/dtab hold that table
+  };
   struct Group {
     BatchLaunchFn bfn;
     int Np, epw, threads;
     size_t lds;
     std::vector<int> members;
-    BatchEntry* dtab = nullptr;
-    // pinned host copy of the table the device holds (the source of its uploads,
-    // kept for the batch's life) and an event after the last upload from it
-    BatchEntry* htab = nullptr;
-    hipEvent_t uploaded = nullptr;
-    bool valid = false;  // htab/dtab hold a table
+    int slot = -1;
     int64_t blocks = 0;
     // members of different Nq in one launch: each keeps the shape it had when the
     // batch was created (threads and lds are the maxima over the members)
@@ -36,6 +42,7 @@ struct atm_batch {
   };
   std::vector<atm_dg*> dgs;
   std::vector<Group> groups;
+  std::vector<Slot> slots;
   hipGraphExec_t exec[2] = {nullptr, nullptr};  // two-step stage sequence per scheme (reads the tables)
 };
 
@@ -60,6 +67,7 @@ struct atm_banded {
   int32_t* elem_col = nullptr;  // column of each element (-1: none)
   int32_t* elem_ez = nullptr;   // position of each element in its column
   int32_t* colelem = nullptr;   // ncols*KV
+  size_t dev_bytes = 0;         // device memory held (factors, block form, index tables)
 };
 
 namespace atm {
@@ -1009,12 +1017,16 @@ Resolved resolve(const atm_dg_desc* d, int Nq, int Np) {
     if (d->law == ATM_LAW_EULER_GRAVITY) r = resolve_fast_euler_gravity(d->dim, sf, Nq);
     if (r.fn) return r;
   }
+  // the Nq-specialised kernels are built for the :per_dir order; the other two
+  // reference orders run the generic kernels (a surface flux no specialisation
+  // is selected by; the kernels read the real one from the operator)
+  const int sel = d->fd_kernel != ATM_FD_PER_DIR ? -1 : sf;
   switch (d->law) {
-    case ATM_LAW_ADVECTION: return resolve_advection(d->dim, f, vf, sf, Nq, Np);
-    case ATM_LAW_EULER: return resolve_euler(d->dim, f, vf, sf, Nq, Np);
-    case ATM_LAW_EULER_GRAVITY: return resolve_euler_gravity(d->dim, f, vf, sf, Nq, Np);
-    case ATM_LAW_LINEAR_EULER_GRAVITY: return resolve_linear_euler_gravity(d->dim, f, vf, sf, Nq, Np);
-    case ATM_LAW_SHALLOW_WATER: return resolve_shallow_water(d->dim, f, vf, sf, Nq, Np);
+    case ATM_LAW_ADVECTION: return resolve_advection(d->dim, f, vf, sel, Nq, Np);
+    case ATM_LAW_EULER: return resolve_euler(d->dim, f, vf, sel, Nq, Np);
+    case ATM_LAW_EULER_GRAVITY: return resolve_euler_gravity(d->dim, f, vf, sel, Nq, Np);
+    case ATM_LAW_LINEAR_EULER_GRAVITY: return resolve_linear_euler_gravity(d->dim, f, vf, sel, Nq, Np);
+    case ATM_LAW_SHALLOW_WATER: return resolve_shallow_water(d->dim, f, vf, sel, Nq, Np);
     default: return Resolved{nullptr, 0, 0, nullptr, 1};
   }
 }
@@ -1048,6 +1060,7 @@ int red_buffer(atm_dg* dg) {
   void* p = nullptr;
   HIPCHK(dg->ctx, hipMalloc(&p, (RED_PART + RED_STATE + RED_OUT) * sizeof(double)));
   dg->allocs.push_back(p);
+  dg->dev_bytes += (RED_PART + RED_STATE + RED_OUT) * sizeof(double);
   dg->red = static_cast<double*>(p);
   return ATM_OK;
 }
@@ -1222,6 +1235,7 @@ int dg_alloc(atm_dg* dg, size_t bytes, void** out) {
   *out = nullptr;
   HIPCHK(dg->ctx, hipMalloc(out, bytes));
   dg->allocs.push_back(*out);
+  dg->dev_bytes += bytes;
   return ATM_OK;
 }
 
@@ -1237,6 +1251,14 @@ int dg_prepare(atm_ctx* ctx, const atm_dg_desc* d, atm_dg** out) {
     return fail(ctx, ATM_ERR_ARG, "unknown volume form");
   if (d->precision != ATM_PRECISION_REFERENCE && d->precision != ATM_PRECISION_FAST)
     return fail(ctx, ATM_ERR_ARG, "unknown precision mode");
+  if (d->fd_kernel < ATM_FD_PER_DIR || d->fd_kernel > ATM_FD_NAIVE)
+    return fail(ctx, ATM_ERR_ARG, "unknown flux-differencing kernel type");
+  if (d->fd_kernel != ATM_FD_PER_DIR && d->volume_form != ATM_FORM_FLUX_DIFFERENCING)
+    return fail(ctx, ATM_ERR_ARG, "a kernel type applies to the flux-differencing form only");
+  if (d->fd_kernel != ATM_FD_PER_DIR && d->precision == ATM_PRECISION_FAST)
+    return fail(ctx, ATM_ERR_ARG, "the kernel type selects a reference summation order; FAST precision has its own");
+  if (d->fd_kernel == ATM_FD_NAIVE && d->dirmask != (1 << d->dim) - 1)
+    return fail(ctx, ATM_ERR_ARG, "the naive flux-differencing kernel needs all directions (src/dgsem.jl:139)");
   const int Nq = d->N + 1;
   int Np = 1;
   for (int i = 0; i < d->dim; ++i) Np *= Nq;
@@ -1293,6 +1315,7 @@ int dg_prepare(atm_ctx* ctx, const atm_dg_desc* d, atm_dg** out) {
   p.dirmask = d->dirmask;
   p.vflux = d->volume_flux;
   p.sflux = d->surface_flux;
+  p.fdk = d->fd_kernel;
   p.Ntot = Ntot;
   p.NF = NF;
   int rc = ATM_OK;
@@ -1428,11 +1451,17 @@ int atm_dg_set_schedule(atm_dg* dg, const int32_t* order, int64_t n) {
   if (dg->lsrk_exec) hipGraphExecDestroy(dg->lsrk_exec);  // captured with the old order
   dg->lsrk_exec = nullptr;
   dg->lsrk_scheme = -1;
-  if (dg->sched) hipFree(dg->sched);
+  if (dg->sched) {
+    hipFree(dg->sched);
+    dg->dev_bytes -= (size_t)dg->dev.K * sizeof(int32_t);
+  }
   dg->sched = dev;
   dg->dev.sched = dev;
+  if (dev) dg->dev_bytes += (size_t)dg->dev.K * sizeof(int32_t);
   return ATM_OK;
 }
+
+int64_t atm_dg_device_bytes(const atm_dg* dg) { return dg ? (int64_t)dg->dev_bytes : ATM_ERR_ARG; }
 
 int atm_dg_epw_candidates(const atm_dg* dg, int32_t* out, int max) {
   if (!dg || (max > 0 && !out)) return ATM_ERR_ARG;
@@ -1718,21 +1747,25 @@ int atm_rlsrk_run(atm_dg* dg, int scheme, double* q, double* work, double dt, do
 }
 
 // ---- batched instances --------------------------------------------------------
-int atm_batch_create(atm_ctx* ctx, int32_t n, atm_dg* const* members, atm_batch** out) {
-  if (!ctx || !out || n < 0 || (n > 0 && !members)) return fail(ctx, ATM_ERR_ARG, "atm_batch_create: bad argument");
-  *out = nullptr;
-  atm_batch* b = new atm_batch();
-  b->ctx = ctx;
+}  // extern "C"
+
+namespace atm {
+
+// (Re)groups a batch's members.  Device and pinned tables stay with the batch:
+// group g takes slot g, which is reallocated only when it is too small.  Graphs
+// captured with the old groups are dropped (after the stream has drained them).
+int batch_set_members(atm_batch* b, int32_t n, atm_dg* const* members) {
+  atm_ctx* ctx = b->ctx;
+  if (n < 0 || (n > 0 && !members)) return fail(ctx, ATM_ERR_ARG, "atm_batch: bad member list");
+  for (int i = 0; i < n; ++i)
+    if (!members[i] || members[i]->ctx != ctx || !members[i]->bfn)
+      return fail(ctx, ATM_ERR_ARG, "atm_batch: member is null, from another context or not batchable");
+  std::vector<atm_batch::Group> groups;
   for (int i = 0; i < n; ++i) {
     atm_dg* dg = members[i];
-    if (!dg || dg->ctx != ctx || !dg->bfn) {
-      delete b;
-      return fail(ctx, ATM_ERR_ARG, "atm_batch_create: member is null, from another context or not batchable");
-    }
-    b->dgs.push_back(dg);
     int g = -1;
-    for (size_t k = 0; k < b->groups.size(); ++k) {
-      const atm_batch::Group& H = b->groups[k];
+    for (size_t k = 0; k < groups.size(); ++k) {
+      const atm_batch::Group& H = groups[k];
       if (dg->mbfn ? (H.multi && H.bfn == dg->mbfn)
                    : (!H.multi && H.bfn == dg->bfn && H.Np == dg->Np && H.epw == dg->epw && H.lds == dg->lds))
         g = (int)k;
@@ -1750,10 +1783,10 @@ int atm_batch_create(atm_ctx* ctx, int32_t n, atm_dg* const* members, atm_batch*
         G.Np = G.epw = G.threads = 0;
         G.lds = 0;
       }
-      b->groups.push_back(G);
-      g = (int)b->groups.size() - 1;
+      groups.push_back(G);
+      g = (int)groups.size() - 1;
     }
-    atm_batch::Group& G = b->groups[g];
+    atm_batch::Group& G = groups[g];
     G.members.push_back(i);
     if (G.multi) {
       G.mepw.push_back(dg->epw);
@@ -1765,11 +1798,11 @@ int atm_batch_create(atm_ctx* ctx, int32_t n, atm_dg* const* members, atm_batch*
   // a multi-degree group launches workgroups of its largest member's size: the
   // other members take as many elements per workgroup as fill it (results do not
   // depend on the shape) while their LDS fits
-  for (auto& G : b->groups) {
+  for (auto& G : groups) {
     if (!G.multi) continue;
     G.lds = 0;
     for (size_t k = 0; k < G.members.size(); ++k) {
-      const atm_dg* dg = b->dgs[G.members[k]];
+      const atm_dg* dg = members[G.members[k]];
       int e = std::max(G.mepw[k], G.threads / dg->Np);
       while (e > G.mepw[k] && dg->lds_fixed + (size_t)e * dg->lds_elem > ctx->max_lds) --e;
       G.mepw[k] = e;
@@ -1778,15 +1811,56 @@ int atm_batch_create(atm_ctx* ctx, int32_t n, atm_dg* const* members, atm_batch*
     }
   }
   hipSetDevice(ctx->device);
-  for (auto& G : b->groups) {
-    const size_t bytes = G.members.size() * sizeof(BatchEntry);
-    hipError_t e = hipMalloc(&G.dtab, bytes);
-    if (e == hipSuccess) e = hipHostMalloc((void**)&G.htab, bytes, hipHostMallocDefault);
-    if (e == hipSuccess) e = hipEventCreateWithFlags(&G.uploaded, hipEventDisableTiming);
-    if (e != hipSuccess) {
-      atm_batch_destroy(b);
-      return hipfail(ctx, e, "batch table allocation");
+  if (b->exec[0] || b->exec[1]) {
+    HIPCHK(ctx, hipStreamSynchronize(stream_of(ctx)));
+    for (hipGraphExec_t& x : b->exec)
+      if (x) hipGraphExecDestroy(x);
+    b->exec[0] = b->exec[1] = nullptr;
+  }
+  if (b->slots.size() < groups.size()) b->slots.resize(groups.size());
+  for (size_t g = 0; g < groups.size(); ++g) {
+    atm_batch::Slot& S = b->slots[g];
+    groups[g].slot = (int)g;
+    const size_t need = groups[g].members.size();
+    if (!S.uploaded) {
+      hipError_t e = hipEventCreateWithFlags(&S.uploaded, hipEventDisableTiming);
+      if (e != hipSuccess) {
+        S.uploaded = nullptr;
+        return hipfail(ctx, e, "batch table event");
+      }
     }
+    if (S.cap >= need) continue;
+    const size_t cap = std::max<size_t>(need, 2 * S.cap);
+    HIPCHK(ctx, hipEventSynchronize(S.uploaded));  // the last upload from the old pinned copy
+    HIPCHK(ctx, hipStreamSynchronize(stream_of(ctx)));  // kernels reading the old device table
+    if (S.dtab) hipFree(S.dtab);
+    if (S.htab) hipHostFree(S.htab);
+    S.dtab = S.htab = nullptr;
+    S.cap = 0;
+    S.valid = false;
+    hipError_t e = hipMalloc(&S.dtab, cap * sizeof(BatchEntry));
+    if (e == hipSuccess) e = hipHostMalloc((void**)&S.htab, cap * sizeof(BatchEntry), hipHostMallocDefault);
+    if (e != hipSuccess) return hipfail(ctx, e, "batch table allocation");
+    S.cap = cap;
+  }
+  b->dgs.assign(members, members + n);
+  b->groups.swap(groups);
+  return ATM_OK;
+}
+
+}  // namespace atm
+
+extern "C" {
+
+int atm_batch_create(atm_ctx* ctx, int32_t n, atm_dg* const* members, atm_batch** out) {
+  if (!ctx || !out || n < 0 || (n > 0 && !members)) return fail(ctx, ATM_ERR_ARG, "atm_batch_create: bad argument");
+  *out = nullptr;
+  atm_batch* b = new atm_batch();
+  b->ctx = ctx;
+  const int rc = batch_set_members(b, n, members);
+  if (rc != ATM_OK) {
+    atm_batch_destroy(b);
+    return rc;
   }
   *out = b;
   return ATM_OK;
@@ -1798,10 +1872,11 @@ int atm_batch_destroy(atm_batch* b) {
   hipStreamSynchronize(stream_of(b->ctx));
   for (hipGraphExec_t& x : b->exec)
     if (x) hipGraphExecDestroy(x);
-  for (auto& G : b->groups) {
-    if (G.dtab) hipFree(G.dtab);
-    if (G.htab) hipHostFree(G.htab);
-    if (G.uploaded) hipEventDestroy(G.uploaded);
+  for (auto& S : b->slots) {
+    if (S.uploaded) hipEventSynchronize(S.uploaded);
+    if (S.dtab) hipFree(S.dtab);
+    if (S.htab) hipHostFree(S.htab);
+    if (S.uploaded) hipEventDestroy(S.uploaded);
   }
   delete b;
   return ATM_OK;
@@ -1825,6 +1900,7 @@ int atm_batch_lsrk_run(atm_batch* b, int scheme, double* const* q, double* const
   // (members' buffers, time steps or shapes); before the host copy is rewritten,
   // the previous upload from it must have finished.
   for (auto& G : b->groups) {
+    atm_batch::Slot& S = b->slots[G.slot];
     std::vector<BatchEntry> tab(G.members.size());
     std::memset(tab.data(), 0, tab.size() * sizeof(BatchEntry));  // padding too: compared bytewise
     int64_t start = 0;
@@ -1846,12 +1922,13 @@ int atm_batch_lsrk_run(atm_batch* b, int scheme, double* const* q, double* const
     }
     G.blocks = start;
     const size_t bytes = tab.size() * sizeof(BatchEntry);
-    if (G.valid && std::memcmp(G.htab, tab.data(), bytes) == 0) continue;
-    HIPCHK(ctx, hipEventSynchronize(G.uploaded));
-    std::memcpy(G.htab, tab.data(), bytes);
-    HIPCHK(ctx, hipMemcpyAsync(G.dtab, G.htab, bytes, hipMemcpyHostToDevice, stream_of(ctx)));
-    HIPCHK(ctx, hipEventRecord(G.uploaded, stream_of(ctx)));
-    G.valid = true;
+    if (S.valid && S.used == tab.size() && std::memcmp(S.htab, tab.data(), bytes) == 0) continue;
+    HIPCHK(ctx, hipEventSynchronize(S.uploaded));
+    std::memcpy(S.htab, tab.data(), bytes);
+    HIPCHK(ctx, hipMemcpyAsync(S.dtab, S.htab, bytes, hipMemcpyHostToDevice, stream_of(ctx)));
+    HIPCHK(ctx, hipEventRecord(S.uploaded, stream_of(ctx)));
+    S.valid = true;
+    S.used = tab.size();
   }
   const LSRKTable t = lsrk_table(scheme);
   int parity = 0;
@@ -1859,7 +1936,8 @@ int atm_batch_lsrk_run(atm_batch* b, int scheme, double* const* q, double* const
     for (int st = 0; st < t.ns; ++st) {
       for (auto& G : b->groups) {
         if (G.blocks == 0) continue;
-        hipError_t e = G.bfn(G.dtab, (int)G.members.size(), G.blocks, G.threads, G.lds, G.epw, stream_of(ctx), parity,
+        hipError_t e = G.bfn(b->slots[G.slot].dtab, (int)G.members.size(), G.blocks, G.threads, G.lds, G.epw,
+                             stream_of(ctx), parity,
                              t.a[st], t.a[st] == 0.0 ? 1 : 0, t.b[st]);
         if (e != hipSuccess) return hipfail(ctx, e, "batched element kernel launch");
       }
@@ -1950,6 +2028,7 @@ int atm_banded_create(atm_dg* lin, int32_t KV, const int32_t* colelem, int32_t n
   if ((e = hipMalloc(&b->elem_col, K * sizeof(int32_t) + 8)) != hipSuccess) return cleanup(hipfail(ctx, e, "hipMalloc"));
   if ((e = hipMalloc(&b->elem_ez, K * sizeof(int32_t) + 8)) != hipSuccess) return cleanup(hipfail(ctx, e, "hipMalloc"));
   if ((e = hipMalloc(&b->colelem, K * sizeof(int32_t) + 8)) != hipSuccess) return cleanup(hipfail(ctx, e, "hipMalloc"));
+  b->dev_bytes += 3 * (K * sizeof(int32_t) + 8);
   if (K > 0) {
     hipMemcpy(b->elem_col, col.data(), K * sizeof(int32_t), hipMemcpyHostToDevice);
     hipMemcpy(b->elem_ez, ez.data(), K * sizeof(int32_t), hipMemcpyHostToDevice);
@@ -2079,6 +2158,7 @@ int atm_banded_create(atm_dg* lin, int32_t KV, const int32_t* colelem, int32_t n
   if ((e = hipMalloc(&b->perm16, perm16.size() * sizeof(int32_t) + 8)) != hipSuccess ||
       (e = hipMalloc(&b->wave_mat16, wmat16.size() * sizeof(int32_t) + 8)) != hipSuccess)
     return done(hipfail(ctx, e, "hipMalloc"));
+  b->dev_bytes += (perm16.size() + wmat16.size()) * sizeof(int32_t) + 16;
   if (!perm16.empty() &&
       ((e = hipMemcpy(b->perm16, perm16.data(), perm16.size() * sizeof(int32_t), hipMemcpyHostToDevice)) != hipSuccess ||
        (e = hipMemcpy(b->wave_mat16, wmat16.data(), wmat16.size() * sizeof(int32_t), hipMemcpyHostToDevice)) !=
@@ -2087,12 +2167,14 @@ int atm_banded_create(atm_dg* lin, int32_t KV, const int32_t* colelem, int32_t n
   if ((e = hipMalloc(&b->perm, perm.size() * sizeof(int32_t) + 8)) != hipSuccess ||
       (e = hipMalloc(&b->wave_mat, wmat.size() * sizeof(int32_t) + 8)) != hipSuccess)
     return done(hipfail(ctx, e, "hipMalloc"));
+  b->dev_bytes += (perm.size() + wmat.size()) * sizeof(int32_t) + 16;
   if (!perm.empty() && ((e = hipMemcpy(b->perm, perm.data(), perm.size() * sizeof(int32_t), hipMemcpyHostToDevice)) != hipSuccess ||
                         (e = hipMemcpy(b->wave_mat, wmat.data(), wmat.size() * sizeof(int32_t), hipMemcpyHostToDevice)) != hipSuccess))
     return done(hipfail(ctx, e, "hipMemcpy"));
   const size_t facsz = (size_t)b->n * b->nb * b->nu;
   if ((e = hipMalloc(&b->fac, facsz * sizeof(double) + 8)) != hipSuccess) return done(hipfail(ctx, e, "hipMalloc factors"));
   if ((e = hipMalloc(&b->line_mat, L * sizeof(int32_t) + 8)) != hipSuccess) return done(hipfail(ctx, e, "hipMalloc"));
+  b->dev_bytes += facsz * sizeof(double) + 8 + L * sizeof(int32_t) + 8;
   if ((e = hipMemcpyAsync(drep, src.data(), src.size() * sizeof(int64_t), hipMemcpyHostToDevice, s)) != hipSuccess ||
       (e = hipMemcpyAsync(b->line_mat, b->line_mat_host.data(), L * sizeof(int32_t), hipMemcpyHostToDevice, s)) !=
           hipSuccess)
@@ -2111,12 +2193,14 @@ int atm_banded_create(atm_dg* lin, int32_t KV, const int32_t* colelem, int32_t n
     const size_t blk_bytes = (size_t)b->nu * b->KV * 4 * (size_t)(b->bw + 1) * (b->bw + 1) * sizeof(double);
     if (band_blk_kernel(lin->Ns, lin->Nq) && blk_bytes <= ((size_t)1 << 31)) {
       if ((e = hipMalloc(&b->blocks, blk_bytes)) != hipSuccess) return done(hipfail(ctx, e, "hipMalloc blocks"));
+      b->dev_bytes += blk_bytes;
       hipLaunchKernelGGL(band_blocks_kernel, dim3(b->nu * b->KV), dim3(64), 0, s, g, b->fac, b->blocks);
       if ((e = hipGetLastError()) != hipSuccess) return done(hipfail(ctx, e, "band blocks launch"));
       if (band_mfma_kernel(lin->Ns, lin->Nq)) {
         const int BS = b->bw + 1, TI = (BS + 15) / 16, KS = (BS + 3) / 4;
         const size_t fbytes = (size_t)b->nu * b->KV * 4 * TI * KS * 64 * sizeof(double);
         if ((e = hipMalloc(&b->frags, fbytes)) != hipSuccess) return done(hipfail(ctx, e, "hipMalloc fragments"));
+        b->dev_bytes += fbytes;
         hipLaunchKernelGGL(band_frags_kernel, dim3(b->nu * b->KV), dim3(256), 0, s, g, b->blocks, b->frags, TI, KS);
         if ((e = hipGetLastError()) != hipSuccess) return done(hipfail(ctx, e, "band fragments launch"));
       }
@@ -2128,6 +2212,8 @@ int atm_banded_create(atm_dg* lin, int32_t KV, const int32_t* colelem, int32_t n
   *out = b;
   return ATM_OK;
 }
+
+int64_t atm_banded_device_bytes(const atm_banded* b) { return b ? (int64_t)b->dev_bytes : ATM_ERR_ARG; }
 
 int atm_banded_destroy(atm_banded* b) {
   if (!b) return ATM_ERR_ARG;

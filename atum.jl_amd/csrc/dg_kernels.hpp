@@ -21,6 +21,7 @@ namespace atm {
 struct DevDG {
   LawConst c;
   int K, Nq, Np, Nfpf, Nfp, naux, dirmask, vflux, sflux;
+  int fdk;          // flux-differencing summation order: ATM_FD_PER_DIR / _SYMMETRIC / _NAIVE
   int64_t Ntot, NF;
   int epw, lds;     // elements per workgroup and LDS bytes of a single-operator launch
   const int32_t* sched;  // processing order of the elements (null: element-id order)
@@ -206,10 +207,18 @@ ATM_DEV void epilogue(const DevDG& p, int64_t node, const double* q1, const doub
 }
 
 // ---------------------------------------------------------------------------
-// Flux-differencing form (:per_dir order), all directions of the mask in one
-// launch.  Per-node primitives of the two-point flux live in LDS for the whole
-// element; the metric row (MJ*g_r) of the direction being summed is staged in
-// LDS one direction at a time.
+// Flux-differencing form, all directions of the mask in one launch, in one of
+// the reference's three summation orders (p.fdk; the Nq-specialised kernels are
+// built for :per_dir only, the others run the generic runtime-Nq kernels):
+//   :per_dir            src/dgsem.jl:470-561, direction after direction, each
+//                       summed over n, then s, then d;
+//   :per_dir_symmetric  src/dgsem.jl:563-681, per direction two sums with
+//                       halved terms, n ascending and n descending, added;
+//   :naive              src/dgsem.jl:308-468, one sum over n with the
+//                       directions interleaved inside it (all directions).
+// Per-node primitives of the two-point flux live in LDS for the whole element;
+// the metric row (MJ*g_r) of the direction being summed is staged in LDS one
+// direction at a time (the naive order reads the neighbours' rows from memory).
 template <class L, int NQ, int VF, int SF>
 ATM_DEV void fd_body(const DevDG& p, double* lds, int64_t blk, const double* __restrict__ q, double* __restrict__ dq, double alpha,
           int assign, double* __restrict__ qout, double bdt, int epw) {
@@ -255,6 +264,54 @@ ATM_DEV void fd_body(const DevDG& p, double* lds, int64_t blk, const double* __r
 #pragma unroll
   for (int s = 0; s < Ns; ++s) dq0[s] = (active && !assign) ? dq[s * p.Ntot + node] : 0.0;
   double acc[Ns];
+  const int fdk = (NQ > 0) ? ATM_FD_PER_DIR : p.fdk;
+  if (fdk == ATM_FD_NAIVE) {
+    // src/dgsem.jl:308-468: sources of all directions, then for each n the pair
+    // of every direction (requires all directions; checked at creation)
+    __syncthreads();  // primitives written
+    if (!active) return;
+    double dl[Ns];
+#pragma unroll
+    for (int s = 0; s < Ns; ++s) dl[s] = -0.0;
+    L::sources(p.c, dl, q1, a1, p.dirmask, false);
+    for (int n = 0; n < Nq; ++n) {
+#pragma unroll
+      for (int r = 0; r < D; ++r) {
+        const int id = idx[r];
+        const int lstride = (r == 0) ? 1 : ((r == 1) ? Nq : Nq * Nq);
+        const int ild = ijk - id * lstride + n * lstride;
+        double P2[NP];
+#pragma unroll
+        for (int k = 0; k < NP; ++k) P2[k] = sP[k * Np + ild];
+        double F[D][Ns];
+        PF::pair(p.c, P1, P2, F);
+        const double Din = MJIn * sD[id + Nq * n];
+        const double Dni = MJIn * sD[n + Nq * id];
+        const int64_t nn = e * Np + ild;
+        const double MJ2 = p.MJ[nn];
+        double lg1[D], lg2[D];
+#pragma unroll
+        for (int d = 0; d < D; ++d) {
+          lg1[d] = MJn * gall[r][d];
+          lg2[d] = MJ2 * p.g[(int64_t)(r + D * d) * p.Ntot + nn];
+        }
+#pragma unroll
+        for (int s = 0; s < Ns; ++s) {
+#pragma unroll
+          for (int d = 0; d < D; ++d) {
+            dl[s] -= Din * lg1[d] * F[d][s];
+            dl[s] += F[d][s] * lg2[d] * Dni;
+          }
+        }
+      }
+    }
+#pragma unroll
+    for (int s = 0; s < Ns; ++s) acc[s] = assign ? dl[s] : alpha * dq0[s] + dl[s];
+    surface_node<L, SF>(p, Nq, e, idx, q, q1, a1, MJIn, acc);
+    epilogue<L>(p, node, q1, acc, dq, qout, bdt);
+    return;
+  }
+  const bool sym = fdk == ATM_FD_PER_DIR_SYMMETRIC;
   bool first = true;
 #pragma unroll
   for (int r = 0; r < D; ++r) {
@@ -288,14 +345,54 @@ ATM_DEV void fd_body(const DevDG& p, double* lds, int64_t blk, const double* __r
       double lgn[D];
 #pragma unroll
       for (int d = 0; d < D; ++d) lgn[d] = slg[d * Np + ild];
+      if (sym) {
 #pragma unroll
-      for (int s = 0; s < Ns; ++s) {
+        for (int s = 0; s < Ns; ++s) {
 #pragma unroll
-        for (int d = 0; d < D; ++d) {
-          dl[s] -= Ddn * lgi[d] * F[d][s];
-          dl[s] += F[d][s] * lgn[d] * Dnd;
+          for (int d = 0; d < D; ++d) {
+            dl[s] -= Ddn * lgi[d] * F[d][s] / 2;
+            dl[s] += F[d][s] * lgn[d] * Dnd / 2;
+          }
+        }
+      } else {
+#pragma unroll
+        for (int s = 0; s < Ns; ++s) {
+#pragma unroll
+          for (int d = 0; d < D; ++d) {
+            dl[s] -= Ddn * lgi[d] * F[d][s];
+            dl[s] += F[d][s] * lgn[d] * Dnd;
+          }
         }
       }
+    }
+    if (sym) {
+      // the second sum of the symmetric order: n descending, from -0.0
+      double dl2[Ns];
+#pragma unroll
+      for (int s = 0; s < Ns; ++s) dl2[s] = -0.0;
+      for (int n = Nq - 1; n >= 0; --n) {
+        const int ild = lbase + n * lstride;
+        double P2[NP];
+#pragma unroll
+        for (int k = 0; k < NP; ++k) P2[k] = sP[k * Np + ild];
+        double F[D][Ns];
+        PF::pair(p.c, P1, P2, F);
+        const double Ddn = MJIn * sD[id + Nq * n];
+        const double Dnd = MJIn * sD[n + Nq * id];
+        double lgn[D];
+#pragma unroll
+        for (int d = 0; d < D; ++d) lgn[d] = slg[d * Np + ild];
+#pragma unroll
+        for (int s = 0; s < Ns; ++s) {
+#pragma unroll
+          for (int d = 0; d < D; ++d) {
+            dl2[s] -= Ddn * lgi[d] * F[d][s] / 2;
+            dl2[s] += F[d][s] * lgn[d] * Dnd / 2;
+          }
+        }
+      }
+#pragma unroll
+      for (int s = 0; s < Ns; ++s) dl[s] = dl[s] + dl2[s];
     }
     if (first) {
 #pragma unroll

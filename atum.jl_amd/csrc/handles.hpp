@@ -49,6 +49,7 @@ struct atm_dg {
   const double* lsrk_scratch = nullptr;
   double lsrk_dt = 0.0;
   std::vector<void*> allocs;
+  size_t dev_bytes = 0;  // device memory the operator holds (allocs, reduction buffer, schedule)
 };
 
 
@@ -96,6 +97,11 @@ inline std::string last_error(atm_ctx* ctx) {
 
 // Device allocation owned by the operator (freed by atm_dg_destroy).
 int dg_alloc(atm_dg* dg, size_t bytes, void** out);
+
+// Regroups a batch for a new member list, keeping its table memory (the serving
+// scheduler changes members whenever requests arrive or finish).  Calls on one
+// batch must be ordered on one stream.
+int batch_set_members(atm_batch* b, int32_t n, atm_dg* const* members);
 
 // Allocation owned by the operator, filled from host memory (count == 0: null).
 template <class T>

@@ -14,6 +14,7 @@
 #include <cmath>
 #include <condition_variable>
 #include <deque>
+#include <map>
 #include <mutex>
 #include <string>
 #include <thread>
@@ -31,7 +32,11 @@ struct Req {
   atm_request r{};
   std::vector<double> q0;  // host copy of the initial state until the upload has landed
   int64_t n = 0;           // doubles in the state
-  uint64_t footprint = 0;  // device bytes while resident
+  uint64_t state_bytes = 0;  // the request's own device buffers while resident
+  // the operators it runs (dg, lin, band; null where unused) and their device bytes:
+  // counted against the budget once, while any resident request runs them
+  const void* ops[3] = {nullptr, nullptr, nullptr};
+  uint64_t op_bytes[3] = {0, 0, 0};
   int64_t left = 0;        // steps still to take
   int state = ATM_REQ_QUEUED;  // published under the server's mutex
   std::string error;           // published with the state
@@ -71,23 +76,115 @@ struct atm_server {
   std::deque<Req*> pending;
   std::deque<int64_t> finished;  // not yet handed out by atm_server_next_done
   std::unordered_map<int64_t, Req*> reqs;
-  uint64_t in_use = 0;
+  uint64_t state_in_use = 0;  // device bytes of resident requests' buffers
+  uint64_t op_in_use = 0;     // device bytes of the operators resident requests run
+  std::unordered_map<const void*, int64_t> op_refs;  // resident requests per operator
+  uint64_t pool_idle = 0;     // published copy of pool_bytes
   int64_t unfinished = 0;
   atm_server_stats stats{};
   // worker-owned
   std::vector<Req*> active;
-  atm_batch* batch[2] = {nullptr, nullptr};  // one per LSRK scheme
+  atm_batch* batch[2] = {nullptr, nullptr};  // one per LSRK scheme, regrouped as members change
   std::vector<int64_t> members[2];
+  // idle device buffers of retired requests, by rounded size, reused by later
+  // admissions instead of hipMalloc/hipFree per request
+  std::multimap<size_t, void*> pool;
+  uint64_t pool_bytes = 0;
+  int64_t pool_hits = 0, pool_misses = 0;
 
   double now() const { return std::chrono::duration<double>(Clock::now() - t0).count(); }
 };
 
 namespace {
 
+constexpr size_t POOL_GRAIN = (size_t)1 << 16;
+size_t pool_round(size_t bytes) { return (bytes + POOL_GRAIN - 1) / POOL_GRAIN * POOL_GRAIN; }
+
+void pool_trim(atm_server* s, uint64_t limit) {
+  while (s->pool_bytes > limit && !s->pool.empty()) {
+    auto it = std::prev(s->pool.end());  // largest first
+    hipFree(it->second);
+    s->pool_bytes -= it->first;
+    s->pool.erase(it);
+  }
+}
+
+// A device buffer of at least `bytes` from the pool (same rounded size) or hipMalloc;
+// on an allocation failure the idle pool is released and the allocation retried.
+hipError_t pool_get(atm_server* s, size_t bytes, double** out) {
+  const size_t rb = pool_round(bytes);
+  auto it = s->pool.find(rb);
+  if (it != s->pool.end()) {
+    *out = static_cast<double*>(it->second);
+    s->pool.erase(it);
+    s->pool_bytes -= rb;
+    s->pool_hits += 1;
+    return hipSuccess;
+  }
+  s->pool_misses += 1;
+  hipError_t e = hipMalloc((void**)out, rb);
+  if (e != hipSuccess) {
+    (void)hipGetLastError();
+    pool_trim(s, 0);
+    e = hipMalloc((void**)out, rb);
+  }
+  return e;
+}
+
+void pool_put(atm_server* s, double* p, size_t bytes) {
+  if (!p) return;
+  const size_t rb = pool_round(bytes);
+  s->pool.emplace(rb, p);
+  s->pool_bytes += rb;
+}
+
+// Buffers back to the pool, once the work on them has finished (after the tick's
+// synchronisation).
+void release_buffers(atm_server* s, Req* r) {
+  const size_t bytes = (size_t)r->n * sizeof(double);
+  pool_put(s, r->q, bytes);
+  pool_put(s, r->dq, bytes);
+  pool_put(s, r->scratch, bytes);
+  pool_put(s, r->work, 9 * bytes);
+  r->q = r->dq = r->scratch = r->work = nullptr;
+}
+
 void free_buffers(Req* r) {
   for (double** p : {&r->q, &r->dq, &r->scratch, &r->work}) {
     if (*p) hipFree(*p);
     *p = nullptr;
+  }
+}
+
+// Device bytes admitting r adds: its buffers and the operators no resident
+// request runs yet (under the server's mutex).
+uint64_t admission_bytes(const atm_server* s, const Req* r) {
+  uint64_t need = r->state_bytes;
+  for (int k = 0; k < 3; ++k) {
+    if (!r->ops[k]) continue;
+    auto it = s->op_refs.find(r->ops[k]);
+    if (it == s->op_refs.end() || it->second == 0) need += r->op_bytes[k];
+  }
+  return need;
+}
+
+void charge(atm_server* s, const Req* r) {
+  s->state_in_use += r->state_bytes;
+  for (int k = 0; k < 3; ++k) {
+    if (!r->ops[k]) continue;
+    if (s->op_refs[r->ops[k]]++ == 0) s->op_in_use += r->op_bytes[k];
+  }
+}
+
+void uncharge(atm_server* s, const Req* r) {
+  s->state_in_use -= r->state_bytes;
+  for (int k = 0; k < 3; ++k) {
+    if (!r->ops[k]) continue;
+    auto it = s->op_refs.find(r->ops[k]);
+    if (it != s->op_refs.end() && --it->second == 0) {
+      s->op_in_use -= r->op_bytes[k];
+      s->op_refs.erase(it);
+    }
   }
 }
 
@@ -105,13 +202,13 @@ int admit_one(atm_server* s, Req* r) {
     hipEventDestroy(r->ready);
     r->ready = nullptr;
   }
-  HIPCHK(ctx, hipMalloc(&r->q, bytes));
+  HIPCHK(ctx, pool_get(s, bytes, &r->q));
   if (r->r.kind == ATM_REQ_LSRK) {
-    HIPCHK(ctx, hipMalloc(&r->dq, bytes));
-    HIPCHK(ctx, hipMalloc(&r->scratch, bytes));
+    HIPCHK(ctx, pool_get(s, bytes, &r->dq));
+    HIPCHK(ctx, pool_get(s, bytes, &r->scratch));
     HIPCHK(ctx, hipMemsetAsync(r->dq, 0, bytes, s->main));
   } else {
-    HIPCHK(ctx, hipMalloc(&r->work, 9 * bytes));
+    HIPCHK(ctx, pool_get(s, 9 * bytes, &r->work));
   }
   HIPCHK(ctx, hipMemcpyAsync(r->q, r->q0.data(), bytes, hipMemcpyHostToDevice, s->main));
   return ATM_OK;
@@ -133,11 +230,13 @@ void run_explicit(atm_server* s, int scheme, const std::vector<Req*>& rs, int64_
     dt.push_back(r->r.dt);
   }
   int rc = ATM_OK;
-  if (!s->batch[scheme] || ids != s->members[scheme]) {
-    if (s->batch[scheme]) atm_batch_destroy(s->batch[scheme]);
-    s->batch[scheme] = nullptr;
-    s->members[scheme].clear();
+  if (!s->batch[scheme]) {
     rc = atm_batch_create(s->ctx, (int32_t)dgs.size(), dgs.data(), &s->batch[scheme]);
+    if (rc == ATM_OK) s->members[scheme] = ids;
+  } else if (ids != s->members[scheme]) {
+    // same batch, new members: its tables keep their memory
+    s->members[scheme].clear();
+    rc = batch_set_members(s->batch[scheme], (int32_t)dgs.size(), dgs.data());
     if (rc == ATM_OK) s->members[scheme] = ids;
   }
   // eager launches, no graph capture: while a stream captures, HIP refuses the
@@ -189,16 +288,21 @@ void worker_loop(atm_server* s) {
     // in submission order while the slots and the budget allow (the first request
     // always fits an empty device: submit rejects requests above the budget)
     while (!s->pending.empty() && (int64_t)(s->active.size() + admitted.size()) < s->desc.max_active &&
-           ((double)(s->in_use + s->pending.front()->footprint) <= s->desc.hbm_budget ||
+           ((double)(s->state_in_use + s->op_in_use + admission_bytes(s, s->pending.front())) <= s->desc.hbm_budget ||
             (s->active.empty() && admitted.empty()))) {
       Req* r = s->pending.front();
       s->pending.pop_front();
-      s->in_use += r->footprint;
+      charge(s, r);
       r->state = ATM_REQ_ACTIVE;
       r->t_admit = tb;
       admitted.push_back(r);
     }
+    s->stats.peak_bytes = std::max(s->stats.peak_bytes, (double)(s->state_in_use + s->op_in_use));
+    s->stats.peak_operator_bytes = std::max(s->stats.peak_operator_bytes, (double)s->op_in_use);
+    // idle pooled buffers and resident requests together stay within the budget
+    const uint64_t pool_limit = (uint64_t)std::max(0.0, s->desc.hbm_budget - (double)(s->state_in_use + s->op_in_use));
     lk.unlock();
+    pool_trim(s, pool_limit);
 
     atm_server_stats d{};
     std::vector<Req*> out;  // leaving this tick (finished or failed)
@@ -289,7 +393,7 @@ void worker_loop(atm_server* s) {
           }
     }
     for (Req* r : out) {
-      free_buffers(r);
+      release_buffers(s, r);
       std::vector<double>().swap(r->q0);
       if (r->ready) {
         hipEventDestroy(r->ready);
@@ -310,7 +414,7 @@ void worker_loop(atm_server* s) {
         s->stats.failed += 1;
       }
       r->t_done = tf;
-      s->in_use -= r->footprint;
+      uncharge(s, r);
       s->unfinished -= 1;
       s->finished.push_back(r->id);
     }
@@ -323,6 +427,9 @@ void worker_loop(atm_server* s) {
     s->stats.enqueue_s += d.enqueue_s;
     s->stats.wait_s += d.wait_s;
     s->stats.retire_s += d.retire_s;
+    s->stats.pool_hits = s->pool_hits;
+    s->stats.pool_misses = s->pool_misses;
+    s->stats.pool_bytes = (double)s->pool_bytes;
     if (!out.empty()) s->cv_done.notify_all();
   }
   tl_stream = nullptr;
@@ -397,6 +504,8 @@ int atm_server_destroy(atm_server* s) {
   hipSetDevice(s->ctx->device);
   for (int k = 0; k < 2; ++k)
     if (s->batch[k]) atm_batch_destroy(s->batch[k]);
+  hipStreamSynchronize(s->main);
+  pool_trim(s, 0);
   for (auto& kv : s->reqs) {
     Req* r = kv.second;
     free_buffers(r);
@@ -443,10 +552,25 @@ int atm_server_submit(atm_server* s, const atm_request* req, int64_t* id) {
   Req* r = new Req();
   r->r = q;
   r->n = (int64_t)shape->Ns * shape->Ntot;
-  r->footprint = (q.kind == ATM_REQ_LSRK ? 3u : 10u) * (uint64_t)r->n * sizeof(double);
-  if ((double)r->footprint > s->desc.hbm_budget) {
+  const size_t sb = (size_t)r->n * sizeof(double);
+  r->state_bytes = q.kind == ATM_REQ_LSRK ? 3 * pool_round(sb) : pool_round(sb) + pool_round(9 * sb);
+  uint64_t total = r->state_bytes;
+  if (q.kind == ATM_REQ_LSRK) {
+    r->ops[0] = q.dg;
+    r->op_bytes[0] = q.dg->dev_bytes;
+  } else {
+    r->ops[0] = q.dg;
+    r->op_bytes[0] = q.dg ? q.dg->dev_bytes : 0;
+    r->ops[1] = q.lin;
+    r->op_bytes[1] = q.lin->dev_bytes;
+    r->ops[2] = q.band;
+    r->op_bytes[2] = (uint64_t)atm_banded_device_bytes(q.band);
+  }
+  for (int k = 0; k < 3; ++k) total += r->op_bytes[k];
+  if ((double)total > s->desc.hbm_budget) {
     delete r;
-    return fail(ctx, ATM_ERR_ARG, "atm_server_submit: the request needs more device memory than the budget");
+    return fail(ctx, ATM_ERR_ARG,
+                "atm_server_submit: the request (its buffers and operators) needs more device memory than the budget");
   }
   r->q0.assign(q.q0, q.q0 + r->n);
   r->left = q.steps;

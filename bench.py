@@ -62,7 +62,10 @@ def bubble_setup(Kd, N, precision="fast"):
                  volume_form=A.FluxDifferencingForm(A.EntropyConservativeFlux()),
                  surface_numericalflux=A.RoeFlux(), precision=precision)
     q0 = P.rising_bubble(grid.points)
-    dt = (1.0 / 3.0) * (2000.0 / Kd) / N / 330.0  # experiment/euler_gravity/risingbubble.jl:66-67
+    # the experiment/euler_gravity/risingbubble.jl:66-67 rule (1/3 dx / N / 330, for the 2-D
+    # bubble) at CFL 1/4: in 3-D its CFL 1/3 is linearly unstable (the 16^3 bubble blows up
+    # between steps 100 and 140); the cost of a stage does not depend on dt
+    dt = 0.25 * (2000.0 / Kd) / N / 330.0
     return grid, dg, q0, dt
 
 
@@ -82,7 +85,7 @@ def cpu_baseline(Kd, N, stages):
               bc="reflect", aux=aux, grav=9.81)
     q = P.rising_bubble(g.points)
     dq = np.zeros_like(q)
-    dt = (1.0 / 3.0) * (2000.0 / Kd) / N / 330.0
+    dt = 0.25 * (2000.0 / Kd) / N / 330.0  # as the timed run
     a = [0.0, -567301805773 / 1357537059087, -2404267990393 / 2016746695238,
          -3550918686646 / 2091501179385, -1275806237668 / 842570457699]
     b = [1432997174477 / 9575080441755, 5161836677717 / 13612068292357,
@@ -98,13 +101,15 @@ def cpu_baseline(Kd, N, stages):
                       f"({el:.1f} s), FP64 C++ oracle with OpenMP over elements"}
 
 
-def ensemble_workload(args):
+def ensemble_workload(args, steps=None):
     """256 shallow-water instances (K ~ U{8..64}^2 elements, N ~ U{3..7}, weak form + Rusanov,
     periodic unit square, seeded Gaussian bumps), LSRK54 with per-instance dt: batched launches
     (one per N group and stage, hipGraph) against launching each instance's kernels alone."""
     import numpy as np
     import atum_hip as A
     import problems as P
+
+    nsteps = args.steps if steps is None else steps
 
     params = P.sw_ensemble_params(n=args.instances)
     dgs, q0s, dts = [], [], []
@@ -117,7 +122,7 @@ def ensemble_workload(args):
         dts.append(P.sw_ensemble_dt(K, N, amp))
     ctx = dgs[0].ctx
     dof = sum(q.size for q in q0s)
-    stages = 5 * args.steps
+    stages = 5 * nsteps
     tuned = None
     if args.ensemble_tune:  # one workgroup shape per launch group, chosen on the batched run itself
         tuned = A.autotune_batch(dgs, q0s, dts)
@@ -126,7 +131,7 @@ def ensemble_workload(args):
     ens.run(2)
     ctx.sync()
     ctx.record(0)
-    ens.run(args.steps)
+    ens.run(nsteps)
     ctx.record(1)
     ms_b = ctx.elapsed_ms(0, 1)
     batched = [q.numpy() for q in qs]
@@ -141,7 +146,7 @@ def ensemble_workload(args):
         ctx.sync()
         ctx.record(2)
         for s_, q in zip(solvers, qs2):
-            s_.run(q, args.steps)
+            s_.run(q, nsteps)
         ctx.record(3)
         per[graph] = ctx.elapsed_ms(2, 3)
         ctx.sync()
@@ -151,15 +156,15 @@ def ensemble_workload(args):
     value = dof * stages / (ms_b / 1e3)
     return {
         "metric": "aggregate FP64 DOF-updates/s per RK stage (batched ensemble)",
-        "value": value, "unit": "DOF-updates/s", "n_gpus": 1, "steps": args.steps, "warmup": 2,
-        "ms_per_step": ms_b / args.steps, "higher_is_better": True, "scaling": "weak",
+        "value": value, "unit": "DOF-updates/s", "n_gpus": 1, "steps": nsteps, "warmup": 2,
+        "ms_per_step": ms_b / nsteps, "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "f64", "data": "synthetic",
         "config": {"workload": f"{len(dgs)} shallow-water instances, K~U{{8..64}}^2, N~U{{3..7}}, "
                                "weak form + Rusanov, LSRK54, per-instance dt (PCG64 seed 20260101)",
                    "dof": dof, "kernel_groups": ens.batch.ngroups,
                    "elements_per_workgroup": {f"N={k[2]}": v[0] for k, v in tuned.items()} if tuned else None},
-        "per_request_eager": {"value": dof * stages / (per[False] / 1e3), "ms_per_step": per[False] / args.steps},
-        "per_request_graph": {"value": dof * stages / (per[True] / 1e3), "ms_per_step": per[True] / args.steps},
+        "per_request_eager": {"value": dof * stages / (per[False] / 1e3), "ms_per_step": per[False] / nsteps},
+        "per_request_graph": {"value": dof * stages / (per[True] / 1e3), "ms_per_step": per[True] / nsteps},
         "speedup_vs_per_request_eager": per[False] / ms_b,
         "speedup_vs_per_request_graph": per[True] / ms_b,
         "bitwise_equal_to_alone": True,
@@ -319,6 +324,23 @@ def serve_workload(args):
     s = stats.summary()
     kinds = {k: sum(r.kind == k for r in reqs) for k in ("vortex", "sw", "imex")}
     per = per_request_sample(reqs[::args.serve_sample], server.cache)
+    # per kind: time to completion, and what serving the kind's requests one after another
+    # would take at the rate its sampled requests reach alone (kinds that fill the device on
+    # their own gain little from shared launches; small ones gain most)
+    import numpy as np
+    per_kind = {}
+    c0 = stats.clock0
+    for k in ("vortex", "sw", "imex"):
+        rk = [r for r in reqs if r.kind == k and r.finished is not None]
+        if not rk:
+            continue
+        lat = np.array([r.finished - (c0 + r.arrival) for r in rk])
+        upd = float(sum(r.dof * r.steps * (3 if k == "imex" else 5) for r in rk))
+        rate = per["per_kind"].get(k, {}).get("value", 0.0)
+        per_kind[k] = {"requests": len(rk), "dof_updates": upd, "p50_s": float(np.percentile(lat, 50)),
+                       "p99_s": float(np.percentile(lat, 99)), "alone_value": rate,
+                       "alone_seconds_estimate": upd / rate if rate > 0 else None}
+    plain_s = sum(v["alone_seconds_estimate"] or 0.0 for v in per_kind.values())
     return {"metric": "served stream: aggregate FP64 DOF-updates/s per RK stage, p50/p99 time-to-completion",
             "value": s["dof_updates_per_s"], "unit": "DOF-updates/s", "n_gpus": 1, "steps": None,
             "warmup": 0, "ms_per_step": None, "higher_is_better": True, "scaling": "weak",
@@ -330,8 +352,10 @@ def serve_workload(args):
             "p50_s": s["p50_s"], "p99_s": s["p99_s"], "mean_s": s["mean_s"], "completed": s["completed"],
             "wall_s": s["wall_s"], "ticks": s["ticks"], "max_active": s["max_active"],
             "breakdown_s": {k: s[k] for k in ("build_s", "admit_s", "enqueue_s", "wait_s", "retire_s", "gpu_span_s")},
-            "per_request_sample": per,
-            "speedup_vs_per_request": s["dof_updates_per_s"] / per["value"] if per["value"] > 0 else None}
+            "per_request_sample": per, "per_kind": per_kind,
+            "plain_seconds_estimate": plain_s,
+            "speedup_vs_per_request": s["dof_updates_per_s"] / per["value"] if per["value"] > 0 else None,
+            "footprint": server.footprint_stats()}
 
 
 def per_request_sample(reqs, cache):
@@ -343,6 +367,7 @@ def per_request_sample(reqs, cache):
 
     ctx = A.context()
     dof_updates, seconds = 0.0, 0.0
+    kinds = {}
     for r in reqs:
         built = r.build(cache)
         if r.kind == "imex":
@@ -360,10 +385,17 @@ def per_request_sample(reqs, cache):
         ctx.record(10)
         solver.run(q, r.steps)
         ctx.record(11)
-        seconds += ctx.elapsed_ms(10, 11) / 1e3
+        el = ctx.elapsed_ms(10, 11) / 1e3
+        seconds += el
         dof_updates += q0.size * r.steps * stages
+        kd = kinds.setdefault(r.kind, [0, 0.0, 0.0])
+        kd[0] += 1
+        kd[1] += q0.size * r.steps * stages
+        kd[2] += el
+    per_kind = {k: {"requests": v[0], "value": v[1] / v[2] if v[2] > 0 else 0.0, "gpu_s": v[2]}
+                for k, v in kinds.items()}
     return {"requests": len(reqs), "value": dof_updates / seconds if seconds > 0 else 0.0,
-            "unit": "DOF-updates/s", "gpu_s": seconds,
+            "unit": "DOF-updates/s", "gpu_s": seconds, "per_kind": per_kind,
             "note": "every k-th request of the stream run alone, one after another (solver launches only)"}
 
 
@@ -417,7 +449,14 @@ def main():
                     help="bubble (default, the metric's configuration); the others are extra lines")
     ap.add_argument("--instances", type=int, default=256)
     ap.add_argument("--weak-K", type=int, default=256, help="weak workload: elements per direction")
-    ap.add_argument("--serve-seconds", type=float, default=10.0)
+    ap.add_argument("--serve-seconds", type=float, default=60.0,
+                    help="serve: length of the Poisson arrival stream (the served configuration: 60 s)")
+    ap.add_argument("--ensemble-steps", type=int, default=100,
+                    help="bubble line: steps of the batched ensemble it carries (the configuration: 100)")
+    ap.add_argument("--no-ensemble", action="store_true",
+                    help="bubble line: leave out the batched shallow-water ensemble object")
+    ap.add_argument("--no-serve", action="store_true",
+                    help="bubble line: leave out the served-stream object")
     ap.add_argument("--chunk", type=int, default=10, help="steps per serving tick")
     ap.add_argument("--epw", type=int, default=0,
                     help="bubble / weak: use this many elements per workgroup instead of tuning")
@@ -485,11 +524,17 @@ def main():
     algo = algorithmic_bytes_per_stage(3, N + 1, K, Ns, naux)
     kern_s = ms / 1e3 / stages
     achieved = algo / kern_s / 1e9
-    traffic = None
+    # HBM bytes per launch come from rocprofv3 FETCH_SIZE / WRITE_SIZE counter runs of this
+    # kernel on this workload (a profiler pass of its own), read here from their summary
+    traffic, traffic_source = None, None
     tpath = os.path.join(ROOT, "profiles", "fd_kernel_bubble16_traffic.json")
-    if os.path.exists(tpath) and Kd == 16 and N == 4:
+    if os.path.exists(tpath) and Kd == 16 and N == 4 and args.precision == "fast":
         try:
-            traffic = json.load(open(tpath)).get("bytes_per_dispatch")
+            tj = json.load(open(tpath))
+            traffic = tj.get("bytes_per_dispatch")
+            traffic_source = {"read_from": "profiles/fd_kernel_bubble16_traffic.json", "measured_in_this_run": False,
+                              "counters": "rocprofv3 --pmc FETCH_SIZE, then WRITE_SIZE, scaled by a copy kernel",
+                              "kernel": tj.get("kernel"), "source_commit": tj.get("source_commit")}
         except Exception:
             traffic = None
     out = {
@@ -507,11 +552,13 @@ def main():
         "data": "synthetic",
         "config": {"workload": f"3-D Euler-gravity rising bubble, {Kd}^3 hex elements, N={N}, "
                                "EC flux differencing + Roe surface, reflective walls, LSRK54",
+                   "time_step": "0.25 dx / N / 330 m/s: CFL 1/4 on the reference bubble's rule (its CFL 1/3 "
+                                "is unstable in 3-D)",
                    "elements": K, "dof": dof, "stages_per_step": 5, "dt": dt,
                    "hipgraph": not args.no_graph, "precision": args.precision,
                    "elements_per_workgroup": dg.epw, "autotune": tuning.get(args.precision)},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "traffic_source": traffic_source,
                      "kernel": "fused stage kernel (volume + surface + LSRK update)",
                      "algorithmic_bytes_per_launch": algo, "avg_launch_us": kern_s * 1e6},
     }
@@ -531,6 +578,20 @@ def main():
                                   "note": "same run with the reference's summation order and no FMA"}
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(Kd, N, args.cpu_stages)
+    # the rest of the metric: the batched ensemble (one launch per stage for all members) and
+    # the served stream (p50/p99 time to completion), each against launching the plain
+    # kernels once per request
+    if not args.no_ensemble:
+        e = ensemble_workload(args, steps=args.ensemble_steps)
+        out["ensemble"] = {k: e[k] for k in ("value", "unit", "steps", "ms_per_step", "config",
+                                             "per_request_eager", "per_request_graph",
+                                             "speedup_vs_per_request_eager", "speedup_vs_per_request_graph",
+                                             "bitwise_equal_to_alone")}
+    if not args.no_serve:
+        sv = serve_workload(args)
+        out["served"] = {k: sv.get(k) for k in ("value", "unit", "config", "p50_s", "p99_s", "mean_s", "completed",
+                                            "wall_s", "max_active", "per_request_sample", "per_kind",
+                                            "speedup_vs_per_request", "footprint")}
     print(json.dumps(out))
 
 

@@ -77,6 +77,15 @@ extern "C" {
 #define ATM_PRECISION_REFERENCE 0
 #define ATM_PRECISION_FAST 1
 
+/* summation order of the flux-differencing volume term in REFERENCE precision:
+ * the `kernel_type` of FluxDifferencingForm (src/dgsem.jl:8-14,135-212).
+ * The three orders compute the same sum; their last bits differ.  NAIVE needs
+ * all directions (as the reference asserts, src/dgsem.jl:139); FAST precision
+ * has its own order and takes PER_DIR here. */
+#define ATM_FD_PER_DIR 0           /* :per_dir,  src/dgsem.jl:470-561 */
+#define ATM_FD_PER_DIR_SYMMETRIC 1 /* :per_dir_symmetric, src/dgsem.jl:563-681 */
+#define ATM_FD_NAIVE 2             /* :naive,    src/dgsem.jl:308-468 */
+
 /* time integrators */
 #define ATM_LSRK54 0  /* src/odesolvers.jl:148-174 */
 #define ATM_LSRK144 1 /* src/odesolvers.jl:176-229 */
@@ -103,6 +112,8 @@ typedef struct atm_dg_desc {
   int32_t naux;           /* aux components per node (0 for laws without aux) */
   int32_t n_bc_states;    /* rows of bc_states (ATM_BC_PRESCRIBED) */
   int32_t precision;      /* ATM_PRECISION_* */
+  int32_t fd_kernel;      /* ATM_FD_* (flux differencing in REFERENCE precision; else 0) */
+  int32_t reserved;       /* 0 */
   double gamma, grav, omega;
   double u[3];            /* advection velocity */
   const double* bc_states;  /* n_bc_states x Ns, host */
@@ -185,6 +196,10 @@ int atm_dg_autotune(atm_dg* dg, const double* q, double* dq, int reps, int32_t* 
 int atm_dg_set_schedule(atm_dg* dg, const int32_t* order, int64_t n);
 /* total nodes Np*K */
 int64_t atm_dg_nnodes(const atm_dg* dg);
+/* device bytes the operator holds (geometry, aux, boundary states, reduction and
+ * schedule buffers), as the serving scheduler counts it against its budget.  No
+ * counterpart in the reference. */
+int64_t atm_dg_device_bytes(const atm_dg* dg);
 
 /* The rhs! protocol: replaces `(dg::DGSEM)(dq, q, time; increment)`
  * (src/dgsem.jl:59-98).  increment != 0: dq += L(q); else dq = L(q).
@@ -217,6 +232,9 @@ int atm_lsrk_run(atm_dg* dg, int scheme, double* q, double* dq, double* scratch,
 int atm_banded_create(atm_dg* lin, int32_t KV, const int32_t* colelem_host, int32_t ncols,
                       double scale, atm_banded** out);
 int atm_banded_destroy(atm_banded* b);
+/* device bytes the factorisation holds (factors, block form, index tables).  No
+ * counterpart in the reference. */
+int64_t atm_banded_device_bytes(const atm_banded* b);
 /* x = A^{-1} b for every line (ldiv!, src/odesolvers.jl:355); x may alias b. */
 int atm_banded_solve(atm_banded* b, double* x, const double* rhs);
 /* Copy the factored band to the host: lines x n x (2*bw+1) in [line][row][bw+col-row]. */
@@ -260,11 +278,16 @@ int atm_batch_lsrk_run(atm_batch* b, int scheme, double* const* q, double* const
 /* ---- serving scheduler (continuous batching on one device) ------------------
  * Simulation requests are submitted from any host thread.  A worker thread of
  * the server admits them in submission order (while fewer than max_active are
- * resident and their device footprint fits hbm_budget), advances every resident
+ * resident and their device footprint fits hbm_budget: a request's own buffers
+ * -- state, solver storage -- plus the device memory of the operators it runs
+ * (atm_dg_device_bytes, atm_banded_device_bytes), an operator counted once while
+ * any resident request runs it), advances every resident
  * request by a common chunk of steps per tick (the LSRK requests in one batched
  * launch per kernel group and stage as atm_batch_lsrk_run, the ARK23 requests on
  * side streams beside it), and retires the finished ones: the final state goes
- * to the request's host result buffer and its device memory is freed.  A
+ * to the request's host result buffer and its buffers return to a pool of idle
+ * buffers that later admissions reuse (trimmed so that pool and resident requests
+ * stay within hbm_budget).  A
  * request's result is bitwise that of its steps run alone with atm_lsrk_run /
  * atm_ark23_run.  The server launches on streams of its own; the context's
  * stream stays with the caller.  Work that a request's operators still need
@@ -315,6 +338,11 @@ typedef struct {
   double enqueue_s;   /* worker: launching the ticks */
   double wait_s;      /* worker: waiting for the device */
   double retire_s;    /* worker: downloading results, freeing */
+  int64_t pool_hits;   /* request buffers taken from the server's pool of idle buffers */
+  int64_t pool_misses; /* request buffers allocated (hipMalloc) */
+  double peak_bytes;   /* peak device bytes counted against hbm_budget (buffers + operators) */
+  double peak_operator_bytes; /* the operators' share of it (each operator counted once) */
+  double pool_bytes;   /* idle pooled buffers now (kept within hbm_budget minus what is resident) */
 } atm_server_stats;
 
 int atm_server_create(atm_ctx* ctx, const atm_server_desc* desc, atm_server** out);

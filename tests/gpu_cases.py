@@ -70,6 +70,7 @@ class Case:
         x = self.grid.points
         rng = np.random.default_rng(seed)
         u = u if u is not None else tuple([1.0, -0.4, 0.7][:dim])
+        self.u = u
         prob = A.Problem(boundary=bc, source=source, omega=omega,
                          bc_states=None if bc_states is None else np.asarray(bc_states))
         self.aux = None

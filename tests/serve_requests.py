@@ -92,3 +92,36 @@ def make_stream(duration=60.0, rate=50.0, seed=20260103, mix=(0.5, 0.3, 0.2), st
         out.append(r)
         rid += 1
     return out
+
+
+def oracle_result(r, cache=None):
+    """The oracle's run of request `r` (same initial state, time step and step
+    count; the operator the request's kind names).  Test infrastructure only."""
+    from oracle import oracle as O
+    built = r.build({} if cache is None else cache)
+    if r.kind == "imex":
+        _, KH, KV, N = r.key
+        ex, _lin, q0, dt = built
+        vx = P.julia_range(0.0, 20e3, KH + 1)
+        vz = P.julia_range(0.0, 10e3, KV + 1)
+        og = O.Grid(N, [vx, vx, vz], [True, True, False], stacked=True)
+        kw = dict(bc="reflect", aux=ex.auxstate, grav=9.81)
+        oex = O.DG("euler_gravity", og, volume_form="fd", volume_flux="ec", surface_flux="rusanov", **kw)
+        olin = O.DG("linear_euler_gravity", og, surface_flux="rusanov", directions=(3,), **kw)
+        q = np.array(q0, dtype=np.float64)
+        if r.steps:
+            O.solve_ark23(oex, olin, q, dt, 1e30, split_rhs=False, max_steps=r.steps)
+        return q
+    kind, K, N = r.key
+    _dg, q0, dt = built
+    q = np.array(q0, dtype=np.float64)
+    if kind == "vortex":
+        v = P.julia_range(-5, 5, K + 1)
+        odg = O.DG("euler", O.Grid(N, [v, v], [True, True]), volume_form="fd", volume_flux="ec",
+                   surface_flux="rusanov")
+    else:
+        v = P.julia_range(0.0, 1.0, K + 1)
+        odg = O.DG("shallow_water", O.Grid(N, [v, v], [True, True]), surface_flux="rusanov", grav=9.81)
+    if r.steps:
+        odg.lsrk_steps(q, dt, r.steps)
+    return q

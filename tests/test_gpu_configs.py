@@ -1,0 +1,180 @@
+"""State-level parity of every workload against the oracle, at reduced size.
+
+Each test runs the workload's own time loop through the C ABI and compares the
+final state with the oracle's run of the same steps on the same inputs:
+
+  * rising bubble (3-D Euler with gravity, EC volume + Roe, reflective walls,
+    LSRK54): 4^3 elements, N = 4, 100 steps, in both arithmetic modes
+    (experiment/euler_gravity/risingbubble.jl:97-100 loop);
+  * shallow-water ensemble (weak form + Rusanov, LSRK54): every member of a
+    batch of degrees N = 3..7 advancing in ONE launch per stage, 100 steps
+    (experiment/shallow_water/bickleyjet.jl:53-54 time step);
+  * 2-D FAST Euler vortices of N = 3..7 in one launch per stage, 100 steps;
+  * IMEX Euler with gravity (ARK23, split_rhs = false, FAST EC explicit
+    operator as the benchmark runs it, banded vertical solve): 2 x 2 x 3
+    stacked elements, N = 3, 100 steps
+    (experiment/euler_gravity/acousticwave.jl:144-148);
+  * the served stream: every vortex, shallow-water and IMEX request served in
+    the mix against the oracle's run of the same request.
+
+Bar: relative max-norm of the state difference <= 1e-10 after the steps.
+"""
+import numpy as np
+import pytest
+
+import problems as P
+from gpu_cases import relmax
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+STATE_TOL = 1e-10
+
+
+@pytest.mark.parametrize("precision", ["reference", "fast"])
+def test_bubble_100_steps(precision):
+    """Time step: CFL 1/4 on the reference's rule (1/3 * dx / N / 330,
+    experiment/euler_gravity/risingbubble.jl:66-67, written for the 2-D bubble).
+    In 3-D that rule's CFL 1/3 is linearly unstable: round-off grows about 5.7x
+    per 10 steps and the 4^3 bubble blows up near step 180, and the reference's
+    own :per_dir and :per_dir_symmetric orders differ by 6e-9 after 100 steps, so
+    no implementation can meet a 1e-10 state bar there.  At CFL 1/4 the same two
+    orders agree to 3e-15 after 200 steps (measured with the oracle)."""
+    import atum_hip as A
+    Kd, N, nsteps = 4, 4, 100
+    v = P.julia_range(0, 2000, Kd + 1)
+    grid = A.brickgrid(A.LobattoCell(N, 3), (v, v, v), (False,) * 3)
+    law = A.EulerGravityLaw(3, grav=9.81, problem=A.Problem(boundary="reflect"))
+    dg = A.DGSEM(law=law, grid=grid, volume_form=A.FluxDifferencingForm(A.EntropyConservativeFlux()),
+                 surface_numericalflux=A.RoeFlux(), precision=precision)
+    q0 = P.rising_bubble(grid.points)
+    dt = 0.25 * (2000 / Kd) / N / 330
+    og = O.Grid(N, [v, v, v], [False] * 3)
+    odg = O.DG("euler_gravity", og, volume_form="fd", volume_flux="ec", surface_flux="roe", bc="reflect",
+               aux=P.flat_aux(og.points, 9.81), grav=9.81)
+    ref = q0.copy()
+    odg.lsrk_steps(ref, dt, nsteps)
+    q = dg.fieldarray(q0)
+    A.LSRK54(dg, q, dt).run(q, nsteps)
+    dg.ctx.sync()
+    got = q.numpy()
+    assert relmax(got, ref) <= STATE_TOL
+    # the bubble has moved: the comparison is not of two copies of the initial state
+    assert relmax(ref, q0) > 1e3 * STATE_TOL
+    # the signal itself (vertical momentum, zero initially) agrees to 1e-8 of its size
+    assert np.abs(got[3] - ref[3]).max() <= 1e-8 * np.abs(ref[3]).max()
+    # mass is conserved to round-off (walls)
+    tot0 = np.sum(og.MJ * q0[0])
+    assert abs(np.sum(og.MJ * got[0]) - tot0) <= 1e-13 * tot0
+
+
+def _sw_member(A, K, N, amp, w, c):
+    v = P.julia_range(0.0, 1.0, K + 1)
+    grid = A.brickgrid(A.LobattoCell(N, 2), (v, v), (True, True))
+    dg = A.DGSEM(law=A.ShallowWaterLaw(2, grav=9.81), grid=grid, surface_numericalflux=A.RusanovFlux())
+    odg = O.DG("shallow_water", O.Grid(N, [v, v], [True, True]), surface_flux="rusanov", grav=9.81)
+    return dg, odg, P.sw_bump(grid.points, amp, w, c), P.sw_ensemble_dt(K, N, amp)
+
+
+def test_sw_ensemble_members_100_steps_against_oracle():
+    """Config-3 style ensemble (draws from the ensemble's own generator, meshes
+    2..6 per side): all degrees in one launch per stage, each member against the
+    oracle after 100 LSRK54 steps."""
+    import atum_hip as A
+    params = P.sw_ensemble_params(n=10, seed=20260101, kmin=2, kmax=6)
+    # every degree of 3..7 present
+    for N in range(3, 8):
+        if N not in {p[1] for p in params}:
+            params.append((3, N, 0.05, 0.1, (0.4, 0.6)))
+    members = [_sw_member(A, *p) for p in params]
+    qs = [m[0].fieldarray(m[2]) for m in members]
+    ens = A.LSRKEnsemble([m[0] for m in members], qs, [m[3] for m in members])
+    assert ens.batch.ngroups == 1
+    ens.run(100)
+    members[0][0].ctx.sync()
+    for (dg, odg, q0, dt), q in zip(members, qs):
+        ref = q0.copy()
+        odg.lsrk_steps(ref, dt, 100)
+        got = q.numpy()
+        assert relmax(got, ref) <= STATE_TOL, (dg.grid.N, dg.grid.K)
+        assert relmax(ref, q0) > 1e3 * STATE_TOL
+
+
+@pytest.mark.parametrize("sflux", ["rusanov", "roe"])
+def test_fast_euler_batch_100_steps_against_oracle(sflux):
+    import atum_hip as A
+    members = []
+    for N, K in ((3, 4), (4, 3), (5, 3), (6, 2), (7, 2)):
+        v = P.julia_range(-5, 5, K + 1)
+        g = A.brickgrid(A.LobattoCell(N, 2), (v, v), (True, True))
+        flux = A.RusanovFlux() if sflux == "rusanov" else A.RoeFlux()
+        dg = A.DGSEM(law=A.EulerLaw(2), grid=g, volume_form=A.FluxDifferencingForm(A.EntropyConservativeFlux()),
+                     surface_numericalflux=flux, precision="fast")
+        odg = O.DG("euler", O.Grid(N, [v, v], [True, True]), volume_form="fd", volume_flux="ec",
+                   surface_flux=sflux)
+        q0 = P.isentropic_vortex(g.points, 0.0, center=(0.0, 0.0), u0=(1.0, 1.0))
+        members.append((dg, odg, q0, 0.1 * (10 / K) / N / 2))
+    qs = [m[0].fieldarray(m[2]) for m in members]
+    ens = A.LSRKEnsemble([m[0] for m in members], qs, [m[3] for m in members])
+    assert ens.batch.ngroups == 1
+    ens.run(100)
+    members[0][0].ctx.sync()
+    for (dg, odg, q0, dt), q in zip(members, qs):
+        ref = q0.copy()
+        odg.lsrk_steps(ref, dt, 100)
+        assert relmax(q.numpy(), ref) <= STATE_TOL, dg.grid.N
+
+
+@pytest.mark.parametrize("precision", ["fast", "reference"])
+def test_imex_100_steps_fast_explicit_operator(precision):
+    """ARK23 (split_rhs = false): the explicit operator is the EC + Rusanov
+    Euler-gravity operator (FAST, as benchmarked, or reference order), the
+    implicit one the linearised vertical operator with its banded solve."""
+    import atum_hip as A
+    N, KH, KV, nsteps = 3, 2, 3, 100
+    vx = P.julia_range(0.0, 20e3, KH + 1)
+    vz = P.julia_range(0.0, 10e3, KV + 1)
+    grid = A.brickgrid(A.LobattoCell(N, 3), (vx, vx, vz), (True, True, False), "stacked")
+    prob = A.Problem(boundary="reflect", referencestate=lambda x: P.isothermal_reference(x))
+    law = A.EulerGravityLaw(3, grav=9.81, problem=prob)
+    ex = A.DGSEM(law=law, grid=grid, volume_form=A.FluxDifferencingForm(A.EntropyConservativeFlux()),
+                 surface_numericalflux=A.RusanovFlux(), precision=precision)
+    lin = A.DGSEM(law=A.LinearEulerGravityLaw(law), grid=grid, surface_numericalflux=A.RusanovFlux(),
+                  directions=(3,), auxstate=ex.auxstate)
+    q0 = P.theta_noise_state(grid, ex.auxstate, 20260102)
+    dt = 0.2 * grid.min_node_distance(dims=(0, 1)) / 330.0
+    og = O.Grid(N, [vx, vx, vz], [True, True, False], stacked=True)
+    kw = dict(bc="reflect", aux=ex.auxstate, grav=9.81)
+    oex = O.DG("euler_gravity", og, volume_form="fd", volume_flux="ec", surface_flux="rusanov", **kw)
+    olin = O.DG("linear_euler_gravity", og, surface_flux="rusanov", directions=(3,), **kw)
+    ref = q0.copy()
+    n, _ = O.solve_ark23(oex, olin, ref, dt, 1e30, split_rhs=False, max_steps=nsteps)
+    assert n == nsteps
+    q = ex.fieldarray(q0)
+    A.ARK23(ex, lin, q, dt, split_rhs=False).run(q, nsteps)
+    ex.ctx.sync()
+    got = q.numpy()
+    assert relmax(got, ref) <= STATE_TOL
+    assert relmax(ref, q0) > 1e3 * STATE_TOL
+    # the noise-driven motion itself agrees to 1e-8 of its size
+    for s in (1, 2, 3):
+        assert np.abs(got[s] - ref[s]).max() <= 1e-8 * np.abs(ref[s]).max()
+
+
+def test_served_results_against_oracle():
+    """Requests of all three kinds served together (continuous batching, IMEX on
+    side streams): each result against the oracle's run of the same request."""
+    import atum_hip as A
+    from serve_requests import make_stream, oracle_result
+    # 15 requests: 8 vortex, 3 shallow water, 4 IMEX; degrees 3..7
+    reqs = make_stream(duration=0.25, rate=60.0, seed=22, steps=(20, 100), vortex_k=(2, 5), sw_k=(2, 5),
+                       imex_kh=(2, 3), imex_kv=3, nrange=(3, 7))
+    kinds = {r.kind for r in reqs}
+    assert kinds == {"vortex", "sw", "imex"}, kinds
+    server = A.Server(chunk=8)
+    stats = server.run(reqs)
+    server.close()
+    assert stats.completed == len(reqs) and stats.max_active >= 2
+    for r in reqs:
+        ref = oracle_result(r)
+        assert relmax(r.result, ref) <= STATE_TOL, (r.rid, r.kind, r.key)

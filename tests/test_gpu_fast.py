@@ -42,8 +42,6 @@ def _tend(dg, q, dq0=None):
 @pytest.mark.parametrize("N", [1, 2, 3, 4, 5, 6, 7])
 @pytest.mark.parametrize("sflux", ["roe", "rusanov", "ec", "matrix"])
 def test_fast_euler(dim, N, sflux):
-    if dim == 3 and N > 5 and sflux == "matrix":
-        pytest.skip("covered by the other surface fluxes at this size")
     c = _fast_case("euler", dim, N, sflux, K=2 if dim == 3 else 3, seed=N)
     ref = c.odg.tendency(c.q)
     assert relmax(_tend(c.dgf, c.q), ref) <= 1e-12

@@ -127,3 +127,65 @@ def test_server_refuses_invalid_requests():
     done = ok.next_done(30.0)
     assert done is r and r.error is None
     ok.close()
+
+
+def _vortex_request(A, rid, dg, grid, steps=6):
+    import problems as P
+    from atum_hip.serve import Request
+    q0 = P.isentropic_vortex(grid.points, 0.0, center=(0.0, 0.0), u0=(1.0, 1.0))
+    return Request(rid, "vortex", ("vortex", rid), steps, 0.0, lambda cache: (dg, q0, 0.02))
+
+
+def _vortex_op(A, K=4, N=3):
+    import problems as P
+    v = P.julia_range(-5, 5, K + 1)
+    grid = A.brickgrid(A.LobattoCell(N, 2), (v, v), (True, True))
+    dg = A.DGSEM(law=A.EulerLaw(2), grid=grid, volume_form=A.FluxDifferencingForm(A.EntropyConservativeFlux()),
+                 surface_numericalflux=A.RusanovFlux(), precision="fast")
+    return dg, grid
+
+
+def _state_bytes(dg):
+    grain = 1 << 16
+    b = dg.Ns * dg.grid.Ntot * 8
+    return 3 * ((b + grain - 1) // grain * grain)
+
+
+def test_admission_counts_operator_memory():
+    """The budget admits a request's buffers plus the device memory of its operators,
+    an operator counted once while resident requests run it: with a budget that two
+    requests' buffers fit but not their two operators as well, the requests on
+    different operators run one at a time, the requests on one operator together;
+    buffers of retired requests are reused from the pool."""
+    import atum_hip as A
+    L = A._lib.lib
+    dga, grid = _vortex_op(A)
+    dgb, _ = _vortex_op(A)
+    sb = _state_bytes(dga)
+    op = L.atm_dg_device_bytes(dga.h)
+    assert op > 0 and op == L.atm_dg_device_bytes(dgb.h)
+    budget = 2 * sb + 2 * op - 1  # both buffers fit; both operators as well do not
+    assert 2 * sb <= budget
+    # different operators: one at a time
+    srv = A.Server(hbm_budget=float(budget), max_active=8, chunk=2)
+    r1, r2 = _vortex_request(A, 1, dga, grid), _vortex_request(A, 2, dgb, grid)
+    st = srv.run([r1, r2])
+    assert st.completed == 2 and st.max_active == 1
+    fp = srv.footprint_stats()
+    assert fp["peak_bytes"] <= budget and fp["peak_operator_bytes"] == op
+    assert fp["pool_hits"] >= 3  # the second request took the first one's three buffers
+    srv.close()
+    # one operator: counted once, so both are resident together
+    srv = A.Server(hbm_budget=float(2 * sb + op), max_active=8, chunk=2)
+    r3, r4 = _vortex_request(A, 3, dga, grid), _vortex_request(A, 4, dga, grid)
+    st = srv.run([r3, r4])
+    assert st.completed == 2 and st.max_active == 2
+    assert srv.footprint_stats()["peak_bytes"] == 2 * sb + op
+    srv.close()
+    for r in (r1, r2, r3, r4):
+        assert np.array_equal(r.result, A.run_alone(r))
+    # buffers alone fit, buffers and operator do not: refused at submit
+    small = A.Server(hbm_budget=float(sb + op - 1))
+    with pytest.raises(A.AtumError, match="budget"):
+        small.submit(_vortex_request(A, 5, dga, grid))
+    small.close()

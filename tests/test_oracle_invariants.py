@@ -182,13 +182,13 @@ def test_semidiscrete_entropy(dim, law):
         assert dgm.entropyproduct(q, dqm) < -1e-3 * scale
 
 
-@pytest.mark.parametrize("law", ["euler", "euler_gravity", "shallow_water"])
-@pytest.mark.parametrize("sf", ["roe", "rusanov", "matrix"])
+# (the reference defines no Matrix flux for shallow water)
+@pytest.mark.parametrize("law,sf", [(law, sf) for law in ("euler", "euler_gravity", "shallow_water")
+                                    for sf in ("roe", "rusanov", "matrix")
+                                    if not (law == "shallow_water" and sf == "matrix")])
 def test_surface_flux_antisymmetry(law, sf):
     """Two elements, periodic: swapping the sides of a face flips the numerical flux, so the
     face contributions of the two elements cancel in the sum of MJ*dq for every state."""
-    if law == "shallow_water" and sf == "matrix":
-        pytest.skip("no Matrix flux for shallow water in the reference")
     v = np.array([0.0, 1.0, 2.0])
     g = O.Grid(2, [v], [True])
     rng = np.random.default_rng(4)
