@@ -100,6 +100,7 @@ _sig("atm_dg_tendency", c_int, [c_void_p, c_void_p, c_void_p, c_double, c_int])
 _sig("atm_lsrk_stage", c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_double, c_double, c_double])
 _sig("atm_lsrk_run", c_int, [c_void_p, c_int, c_void_p, c_void_p, c_void_p, c_double, c_double,
                              c_int64, c_int])
+_sig("atm_lsrk_run_noalloc", c_int, [c_void_p, c_int, c_void_p, c_void_p, c_double, c_double, c_int64])
 _sig("atm_banded_create", c_int, [c_void_p, c_int32, _i32p, c_int32, c_double,
                                   ctypes.POINTER(c_void_p)])
 _sig("atm_banded_destroy", c_int, [c_void_p])

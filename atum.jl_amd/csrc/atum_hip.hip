@@ -1088,6 +1088,34 @@ int launch(atm_dg* dg, const double* q, double* dq, double alpha, int assign, do
   return ATM_OK;
 }
 
+// ---- fixed scratch of the no-allocation LSRK entry point ----------------------
+// The ping-pong state of atm_lsrk_run_noalloc lives in this file-scope device
+// array: every device the library runs on holds its own copy from module load
+// until the process ends.  In host code the array's name is a host-side stand-in;
+// host code uses only the device address hipGetSymbolAddress returns for it on
+// each device (resolved once per device when a context is created there).
+constexpr int64_t NOALLOC_MAX_DOUBLES = (int64_t)8 << 20;  // 64 MiB
+static __device__ __align__(256) double scratch[NOALLOC_MAX_DOUBLES];
+
+constexpr int NOALLOC_MAX_DEVICES = 64;
+std::mutex noalloc_mu;
+bool noalloc_resolved[NOALLOC_MAX_DEVICES] = {};
+double* noalloc_ptr[NOALLOC_MAX_DEVICES] = {};
+
+// with `device` current: its copy's address (null if it cannot be resolved)
+double* noalloc_scratch_of(int device) {
+  if (device < 0 || device >= NOALLOC_MAX_DEVICES) return nullptr;
+  std::lock_guard<std::mutex> lk(noalloc_mu);
+  if (!noalloc_resolved[device]) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    noalloc_ptr[device] = (e == hipSuccess) ? static_cast<double*>(p) : nullptr;
+    if (e != hipSuccess) (void)hipGetLastError();
+    noalloc_resolved[device] = true;
+  }
+  return noalloc_ptr[device];
+}
+
 }  // namespace
 
 // =============================================================================
@@ -1108,6 +1136,7 @@ int atm_ctx_create(int device, atm_ctx** out) {
     return ATM_ERR_HIP;
   }
   c->stream = c->own;
+  c->noalloc_scratch = noalloc_scratch_of(device);
   int lds = 0;
   if (hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, device) == hipSuccess && lds > 0)
     c->max_lds = (size_t)lds;
@@ -1614,6 +1643,37 @@ int atm_lsrk_run(atm_dg* dg, int scheme, double* q, double* dq, double* scratch,
     const int64_t n = (int64_t)dg->Ns * dg->Ntot;
     hipLaunchKernelGGL(copy_kernel, dim3(ew_blocks(n)), dim3(EW_THREADS), 0, stream_of(dg->ctx), q, scratch, n);
     HIPCHK(dg->ctx, hipGetLastError());
+  }
+  return ATM_OK;
+}
+
+int atm_lsrk_run_noalloc(atm_dg* dg, int scheme, double* q, double* dq, double dt, double t0, int64_t nsteps) {
+  (void)t0;
+  if (!dg || !q || !dq) return fail(dg ? dg->ctx : nullptr, ATM_ERR_ARG, "atm_lsrk_run_noalloc: null argument");
+  atm_ctx* ctx = dg->ctx;
+  if (scheme != ATM_LSRK54 && scheme != ATM_LSRK144) return fail(ctx, ATM_ERR_ARG, "unknown LSRK scheme");
+  if (q == dq) return fail(ctx, ATM_ERR_ARG, "atm_lsrk_run_noalloc: buffers alias");
+  if (nsteps < 0) return fail(ctx, ATM_ERR_ARG, "atm_lsrk_run_noalloc: nsteps < 0");
+  const int64_t n = (int64_t)dg->Ns * dg->Ntot;
+  if (n > NOALLOC_MAX_DOUBLES)
+    return fail(ctx, ATM_ERR_ARG,
+                "atm_lsrk_run_noalloc: the state has " + std::to_string(n) + " doubles, more than the " +
+                    std::to_string(NOALLOC_MAX_DOUBLES) + " of the fixed scratch (use atm_lsrk_run)");
+  double* sc = ctx->noalloc_scratch;
+  if (!sc) return fail(ctx, ATM_ERR_HIP, "atm_lsrk_run_noalloc: no device address for the fixed scratch");
+  HIPCHK(ctx, hipSetDevice(ctx->device));
+  const LSRKTable t = lsrk_table(scheme);
+  double* bufs[2] = {q, sc};
+  int cur = 0;  // buffer holding the current state
+  for (int64_t step = 0; step < nsteps; ++step)
+    for (int st = 0; st < t.ns; ++st) {
+      const int rc = launch(dg, bufs[cur], dq, t.a[st], t.a[st] == 0.0 ? 1 : 0, bufs[1 - cur], t.b[st] * dt);
+      if (rc) return rc;
+      cur = 1 - cur;
+    }
+  if (cur != 0) {
+    hipLaunchKernelGGL(copy_kernel, dim3(ew_blocks(n)), dim3(EW_THREADS), 0, stream_of(ctx), q, sc, n);
+    HIPCHK(ctx, hipGetLastError());
   }
   return ATM_OK;
 }

@@ -23,6 +23,9 @@ struct atm_ctx {
   // snapshots: a second stream (default priority) for device-to-host copies
   hipStream_t copy = nullptr;
   hipEvent_t snap_ready[16] = {}, snap_done[16] = {};
+  // device address of the library's fixed no-allocation scratch array on this
+  // context's device (null if it could not be resolved)
+  double* noalloc_scratch = nullptr;
 };
 
 struct atm_dg {

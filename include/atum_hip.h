@@ -221,6 +221,20 @@ int atm_lsrk_stage(atm_dg* dg, const double* q_in, double* q_out, double* dq, do
 int atm_lsrk_run(atm_dg* dg, int scheme, double* q, double* dq, double* scratch, double dt,
                  double t0, int64_t nsteps, int use_graph);
 
+/* atm_lsrk_run without the scratch argument, for callers that cannot allocate
+ * device memory (code under HIP graph capture, tight serving loops, harnesses
+ * that forbid hipMalloc).  No counterpart in the reference.  The ping-pong state
+ * lives in a fixed device array of the library (64 MiB, 8,388,608 doubles, held
+ * on every device the library runs on until the process ends); the call
+ * allocates nothing, launches the same stage kernels eagerly (no graph
+ * instantiation) and gives bit-for-bit the result of atm_lsrk_run.  States of
+ * more than 8,388,608 doubles (Ns*Np*K) are refused with ATM_ERR_ARG.  Every
+ * call on one device shares that array: calls that overlap on one GPU (other
+ * streams, other host threads, other contexts) race and must be ordered by the
+ * caller. */
+int atm_lsrk_run_noalloc(atm_dg* dg, int scheme, double* q, double* dq, double dt, double t0,
+                         int64_t nsteps);
+
 /* ---- banded vertical solve (src/bandedsystem.jl + Bennu banded LU / ldiv!) ---
  * Assembles A = I + scale*L for the linear vertical operator `lin` (its
  * dirmask must be the last direction only) by probing it with coloured unit
