@@ -26,6 +26,11 @@ for p in (os.path.join(ROOT, "atum.jl_amd"), ROOT, os.path.join(ROOT, "tests")):
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
 
 
+def progress(msg):
+    """A progress line on stderr (the JSON result stays alone on stdout)."""
+    print(f"bench: {msg}", file=sys.stderr, flush=True)
+
+
 def algorithmic_bytes_per_stage(dim, Nq, K, Ns, naux):
     """Canonical traffic of one stage in the reference data model: per node q and dq
     read+write (32*Ns), metric + MJ ((d^2+1)*8), aux (naux*8); per face point normal +
@@ -318,11 +323,13 @@ def serve_workload(args):
     from serve_requests import make_stream
 
     reqs = make_stream(duration=args.serve_seconds)
+    progress(f"serve: {len(reqs)} requests")
     server = A.Server(chunk=args.chunk, keep_results=False, imex_streams=args.imex_streams,
                       tune=A.TuneCache() if args.serve_tune else None)
     stats = server.run(reqs)
     s = stats.summary()
     kinds = {k: sum(r.kind == k for r in reqs) for k in ("vortex", "sw", "imex")}
+    progress(f"serve: served in {stats.wall:.1f} s; per-request sample (every {args.serve_sample}th alone)")
     per = per_request_sample(reqs[::args.serve_sample], server.cache)
     # per kind: time to completion, and what serving the kind's requests one after another
     # would take at the rate its sampled requests reach alone (kinds that fill the device on
@@ -515,7 +522,9 @@ def main():
             raise SystemExit("bench: non-finite state")
         return grid, dg, dt, ms
 
+    progress(f"bubble {Kd}^3 N={N} ({args.precision}): tuning and {args.steps} timed steps")
     grid, dg, dt, ms = timed(args.precision)
+    progress(f"bubble: {ms / args.steps:.3f} ms per step")
     Ns, K, Np = dg.Ns, grid.K, grid.Np
     stages = 5 * args.steps
     dof = Ns * Np * K
@@ -570,6 +579,7 @@ def main():
                                     "ordered pair, diagonal included); the fused kernel evaluates each "
                                     "unordered pair once and also does the surface term and the update"}
     if args.precision == "fast" and not args.no_reference_order:
+        progress("bubble, reference-order kernel")
         _, _, _, ms_ref = timed("reference")
         out["reference_order"] = {"value": dof * stages / (ms_ref / 1e3), "unit": "DOF-updates/s",
                                   "ms_per_step": ms_ref / args.steps,
@@ -577,17 +587,20 @@ def main():
                                   "autotune": tuning.get("reference"),
                                   "note": "same run with the reference's summation order and no FMA"}
     if not args.no_cpu_baseline:
+        progress(f"CPU baseline: {args.cpu_stages} stages of the oracle")
         out["cpu_baseline"] = cpu_baseline(Kd, N, args.cpu_stages)
     # the rest of the metric: the batched ensemble (one launch per stage for all members) and
     # the served stream (p50/p99 time to completion), each against launching the plain
     # kernels once per request
     if not args.no_ensemble:
+        progress(f"ensemble: {args.instances} shallow-water instances, {args.ensemble_steps} steps")
         e = ensemble_workload(args, steps=args.ensemble_steps)
         out["ensemble"] = {k: e[k] for k in ("value", "unit", "steps", "ms_per_step", "config",
                                              "per_request_eager", "per_request_graph",
                                              "speedup_vs_per_request_eager", "speedup_vs_per_request_graph",
                                              "bitwise_equal_to_alone")}
     if not args.no_serve:
+        progress(f"served stream: {args.serve_seconds:.0f} s of Poisson arrivals")
         sv = serve_workload(args)
         out["served"] = {k: sv.get(k) for k in ("value", "unit", "config", "p50_s", "p99_s", "mean_s", "completed",
                                             "wall_s", "max_active", "per_request_sample", "per_kind",
