@@ -21,6 +21,6 @@ from .odesolvers import ARK23, LSRK54, LSRK144, RLSRK54, RLSRK144, BandedOperato
 from .batch import Batch, LSRKEnsemble  # noqa: F401
 from .serve import Request, Server, run_alone  # noqa: F401
 from .snapshots import SnapshotWriter, load_checkpoint, read_snapshot, write_snapshot  # noqa: F401
-from .tuning import TuneCache, autotune, autotune_batch  # noqa: F401
+from .tuning import TuneCache, autotune, autotune_banded, autotune_batch  # noqa: F401
 
 __all__ = [n for n in dir() if not n.startswith("_")]

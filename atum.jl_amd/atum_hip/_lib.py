@@ -109,6 +109,10 @@ _sig("atm_banded_shape", c_int, [c_void_p, _i64p, _i32p, _i32p])
 _sig("atm_banded_download", c_int, [c_void_p, _dp])
 _sig("atm_banded_nunique", c_int64, [c_void_p])
 _sig("atm_banded_device_bytes", c_int64, [c_void_p])
+_sig("atm_banded_set_kernel", c_int, [c_void_p, c_int])
+_sig("atm_banded_get_kernel", c_int, [c_void_p])
+_sig("atm_banded_autotune", c_int, [c_void_p, c_void_p, c_void_p, c_int, _i32p, ctypes.POINTER(ctypes.c_float), c_int])
+BAND_KERNELS = {"auto": -1, "mfma": 0, "block": 1, "reg": 2, "lds": 3, "plain": 4}
 _sig("atm_ark23_run", c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_double, c_double,
                               c_int64, c_int, c_int])
 _sig("atm_event_record", c_int, [c_void_p, c_int])

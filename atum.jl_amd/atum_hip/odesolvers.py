@@ -167,6 +167,28 @@ class BandedOperator:
         """x = A \\ rhs (x may be rhs)."""
         self.linrhs.ctx.check(_lib.lib.atm_banded_solve(self.h, x.ptr, rhs.ptr), "atm_banded_solve")
 
+    @property
+    def kernel(self) -> str:
+        """The solve kernel a solve uses ("mfma", "block", "reg", "lds" or "plain")."""
+        k = _lib.lib.atm_banded_get_kernel(self.h)
+        return {v: n for n, v in _lib.BAND_KERNELS.items()}[k]
+
+    @kernel.setter
+    def kernel(self, name: str):
+        self.linrhs.ctx.check(_lib.lib.atm_banded_set_kernel(self.h, _lib.BAND_KERNELS[name]),
+                              "atm_banded_set_kernel")
+
+    def autotune(self, x: Field, rhs: Field, reps: int = 10):
+        """Time every solve kernel that can run these factors, keep the fastest; returns
+        (name, {name: microseconds per solve}).  x is overwritten."""
+        ks = (ctypes.c_int32 * 8)()
+        us = (ctypes.c_float * 8)()
+        best = _lib.lib.atm_banded_autotune(self.h, x.ptr, rhs.ptr, int(reps), ks, us, 8)
+        self.linrhs.ctx.check(best if best < 0 else 0, "atm_banded_autotune")
+        names = {v: n for n, v in _lib.BAND_KERNELS.items()}
+        times = {names[ks[i]]: float(us[i]) for i in range(5) if names.get(ks[i]) and us[i] > 0}
+        return names[best], times
+
     def host(self):
         """The factors of every line, (nlines, n, 2bw+1), band storage."""
         n = ctypes.c_int32(0)

@@ -1,12 +1,17 @@
-"""Workgroup-shape autotuner for the element kernels.
+"""Autotuner of launch shapes: elements per workgroup of the element kernels and
+the solve kernel of the banded vertical solve.
 
 The number of elements per workgroup (epw) of an operator's element kernel
 changes how the device is filled, never the results (each element is computed
 by its own threads in a fixed order).  `autotune` times the shapes that fit
 (`atm_dg_autotune`: tendency launches timed with HIP events) and keeps the
-fastest; a `TuneCache` remembers the choice per kernel configuration and mesh
-size class, in memory or in a JSON file, so later operators of the same kind
-skip the timing.
+fastest.  The banded solve has five kernels of different tile shapes (16-line
+block form on the matrix cores or the vector units, 64-line register-blocked
+substitution, LDS and plain substitution) whose availability and speed depend
+on the degree; `autotune_banded` times the ones that can run a factorisation
+(`atm_banded_autotune`) and keeps the fastest.  A `TuneCache` remembers each
+choice per kernel configuration and size class, in memory or in a JSON file, so
+later operators of the same kind skip the timing.
 """
 from __future__ import annotations
 
@@ -42,7 +47,10 @@ class TuneCache:
         return self.table.get(self.key(dg))
 
     def put(self, dg, epw: int):
-        self.table[self.key(dg)] = int(epw)
+        self.store(self.key(dg), int(epw))
+
+    def store(self, key: str, value):
+        self.table[key] = value
         if self.path:
             tmp = self.path + ".tmp"
             with open(tmp, "w") as f:
@@ -117,3 +125,31 @@ def autotune_batch(dgs, q0s, dts, steps: int = 2, rounds: int = 1):
                 m.epw = best
             out[key] = (best, times)
     return out
+
+
+def banded_key(band) -> str:
+    """Tuning key of a banded factorisation: law, degree, elements per column and the
+    size class of its line count."""
+    lin = band.linrhs
+    _, KV, ncols = lin.grid.columns()
+    return json.dumps(["banded", lin.law.law, lin.grid.dim, lin.grid.N, int(KV),
+                       _size_class(int(ncols) * (lin.grid.Np // lin.grid.Nq))])
+
+
+def autotune_banded(band, reps: int = 10, cache: TuneCache | None = None):
+    """Choose the banded solve kernel by timing solves of a right-hand side of ones;
+    returns (kernel name, {name: us per solve}) (timings empty on a cache hit)."""
+    key = banded_key(band)
+    if cache is not None and key in cache.table:
+        try:
+            band.kernel = cache.table[key]
+            return band.kernel, {}
+        except _lib.AtumError:
+            pass
+    lin = band.linrhs
+    rhs = lin.fieldarray()
+    rhs.fill_(1.0)
+    best, times = band.autotune(lin.fieldarray(), rhs, reps=reps)
+    if cache is not None:
+        cache.store(key, best)
+    return best, times

@@ -68,6 +68,7 @@ struct atm_banded {
   int32_t* elem_ez = nullptr;   // position of each element in its column
   int32_t* colelem = nullptr;   // ncols*KV
   size_t dev_bytes = 0;         // device memory held (factors, block form, index tables)
+  int kernel = ATM_BAND_AUTO;   // solve kernel (ATM_BAND_*)
 };
 
 namespace atm {
@@ -1811,6 +1812,8 @@ int atm_rlsrk_run(atm_dg* dg, int scheme, double* q, double* work, double dt, do
 
 namespace atm {
 
+constexpr int MULTI_MAX_THREADS = 512;  // launch bound of the multi-degree batched kernels
+
 // (Re)groups a batch's members.  Device and pinned tables stay with the batch:
 // group g takes slot g, which is reallocated only when it is too small.  Graphs
 // captured with the old groups are dropped (after the stream has drained them).
@@ -1849,10 +1852,15 @@ int batch_set_members(atm_batch* b, int32_t n, atm_dg* const* members) {
     atm_batch::Group& G = groups[g];
     G.members.push_back(i);
     if (G.multi) {
-      G.mepw.push_back(dg->epw);
-      G.mlds.push_back((int)dg->lds);
-      G.threads = std::max(G.threads, ((dg->epw * dg->Np + 63) / 64) * 64);
-      G.lds = std::max(G.lds, dg->lds);
+      // the multi-degree kernels are built for workgroups of up to 512 threads: a
+      // member whose own shape is larger (3-D N = 6 at two elements per workgroup)
+      // takes fewer elements per workgroup here
+      int e0 = dg->epw;
+      while (e0 > 1 && ((e0 * dg->Np + 63) / 64) * 64 > MULTI_MAX_THREADS) --e0;
+      G.mepw.push_back(e0);
+      G.mlds.push_back((int)(dg->lds_fixed + (size_t)e0 * dg->lds_elem));
+      G.threads = std::max(G.threads, ((e0 * dg->Np + 63) / 64) * 64);
+      G.lds = std::max(G.lds, dg->lds_fixed + (size_t)e0 * dg->lds_elem);
     }
   }
   // a multi-degree group launches workgroups of its largest member's size: the
@@ -2292,38 +2300,144 @@ int atm_banded_destroy(atm_banded* b) {
   return ATM_OK;
 }
 
-int atm_banded_solve(atm_banded* b, double* x, const double* rhs) {
-  if (!b || !x || !rhs) return ATM_ERR_ARG;
+}  // extern "C"
+
+namespace {
+
+// Whether solve kernel k can run this factorisation (operand tables built, the
+// (Ns, Nq) instantiated, LDS large enough).
+bool band_kernel_ok(const atm_banded* b, int k) {
+  const atm_ctx* ctx = b->lin->ctx;
+  switch (k) {
+    case ATM_BAND_MFMA:
+      return b->frags && band_mfma_kernel(b->lin->Ns, b->lin->Nq) && band_mfma_lds_bytes(b->n, b->KV) <= ctx->max_lds;
+    case ATM_BAND_BLOCK:
+      return b->blocks && band_blk_kernel(b->lin->Ns, b->lin->Nq);
+    case ATM_BAND_REG:
+      return band_reg_kernel(b->lin->Ns, b->lin->Nq) != nullptr;
+    case ATM_BAND_LDS:
+      return band_solve_lds_bytes(b->n) <= ctx->max_lds;
+    case ATM_BAND_PLAIN:
+      return true;
+    default:
+      return false;
+  }
+}
+
+// The kernel the automatic choice takes: the first available in the order of
+// measured speed on the configurations here (MFMA block form, register block
+// form, register-blocked substitution, LDS substitution, plain substitution).
+int band_auto_kernel(const atm_banded* b) {
+  for (int k : {ATM_BAND_MFMA, ATM_BAND_BLOCK, ATM_BAND_REG, ATM_BAND_LDS})
+    if (band_kernel_ok(b, k)) return k;
+  return ATM_BAND_PLAIN;
+}
+
+int band_launch(atm_banded* b, int k, double* x, const double* rhs) {
   atm_ctx* ctx = b->lin->ctx;
   const BandGeo g = band_geo(b);
-  const size_t lds = band_solve_lds_bytes(b->n);
-  BandSolveMfmaFn mfn = b->frags ? band_mfma_kernel(b->lin->Ns, b->lin->Nq) : nullptr;
-  BandSolveBlkFn bfn = b->blocks ? band_blk_kernel(b->lin->Ns, b->lin->Nq) : nullptr;
-  if (mfn && band_mfma_lds_bytes(b->n, b->KV) <= ctx->max_lds) {
-    if (b->nwaves16 > 0)
-      hipLaunchKernelGGL(mfn, dim3(b->nwaves16), dim3(64), band_mfma_lds_bytes(b->n, b->KV), stream_of(ctx), g,
-                         b->frags, b->perm16, b->wave_mat16, x, rhs);
-  } else if (bfn) {
-    if (b->nwaves16 > 0)
-      hipLaunchKernelGGL(bfn, dim3(b->nwaves16), dim3(64), 0, stream_of(ctx), g, b->blocks, b->perm16, b->wave_mat16, x,
-                         rhs);
-  } else if (BandSolveRegFn fn = band_reg_kernel(b->lin->Ns, b->lin->Nq)) {
-    if (b->nwaves > 0)
-      hipLaunchKernelGGL(fn, dim3(b->nwaves), dim3(64), 0, stream_of(ctx), g, b->fac, b->perm, b->wave_mat, x, rhs);
-  } else if (lds <= ctx->max_lds) {
-    constexpr int LINES = 64 / BAND_SOLVE_PARTS;
-    const int64_t blocks = (b->nlines + LINES - 1) / LINES;
-    if (blocks > 0)
-      hipLaunchKernelGGL(band_solve_lds_kernel<BAND_SOLVE_PARTS>, dim3(blocks), dim3(64), lds, stream_of(ctx), g,
-                         b->fac, b->line_mat, b->nu, x, rhs);
-  } else {
-    const int64_t blocks = (b->nlines + 63) / 64;
-    if (blocks > 0)
-      hipLaunchKernelGGL(band_solve_kernel, dim3(blocks), dim3(64), 0, stream_of(ctx), g, b->fac, b->line_mat, b->nu, x,
-                         rhs);
+  hipStream_t st = stream_of(ctx);
+  switch (k) {
+    case ATM_BAND_MFMA:
+      if (b->nwaves16 > 0)
+        hipLaunchKernelGGL(band_mfma_kernel(b->lin->Ns, b->lin->Nq), dim3(b->nwaves16), dim3(64),
+                           band_mfma_lds_bytes(b->n, b->KV), st, g, b->frags, b->perm16, b->wave_mat16, x, rhs);
+      break;
+    case ATM_BAND_BLOCK:
+      if (b->nwaves16 > 0)
+        hipLaunchKernelGGL(band_blk_kernel(b->lin->Ns, b->lin->Nq), dim3(b->nwaves16), dim3(64), 0, st, g, b->blocks,
+                           b->perm16, b->wave_mat16, x, rhs);
+      break;
+    case ATM_BAND_REG:
+      if (b->nwaves > 0)
+        hipLaunchKernelGGL(band_reg_kernel(b->lin->Ns, b->lin->Nq), dim3(b->nwaves), dim3(64), 0, st, g, b->fac,
+                           b->perm, b->wave_mat, x, rhs);
+      break;
+    case ATM_BAND_LDS: {
+      constexpr int LINES = 64 / BAND_SOLVE_PARTS;
+      const int64_t blocks = (b->nlines + LINES - 1) / LINES;
+      if (blocks > 0)
+        hipLaunchKernelGGL(band_solve_lds_kernel<BAND_SOLVE_PARTS>, dim3(blocks), dim3(64), band_solve_lds_bytes(b->n),
+                           st, g, b->fac, b->line_mat, b->nu, x, rhs);
+      break;
+    }
+    default: {
+      const int64_t blocks = (b->nlines + 63) / 64;
+      if (blocks > 0)
+        hipLaunchKernelGGL(band_solve_kernel, dim3(blocks), dim3(64), 0, st, g, b->fac, b->line_mat, b->nu, x, rhs);
+      break;
+    }
   }
   HIPCHK(ctx, hipGetLastError());
   return ATM_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int atm_banded_solve(atm_banded* b, double* x, const double* rhs) {
+  if (!b || !x || !rhs) return ATM_ERR_ARG;
+  return band_launch(b, b->kernel == ATM_BAND_AUTO ? band_auto_kernel(b) : b->kernel, x, rhs);
+}
+
+int atm_banded_get_kernel(const atm_banded* b) {
+  return b ? (b->kernel == ATM_BAND_AUTO ? band_auto_kernel(b) : b->kernel) : ATM_ERR_ARG;
+}
+
+int atm_banded_set_kernel(atm_banded* b, int kernel) {
+  if (!b) return ATM_ERR_ARG;
+  if (kernel == ATM_BAND_AUTO) {
+    b->kernel = kernel;
+    return ATM_OK;
+  }
+  if (kernel < ATM_BAND_MFMA || kernel > ATM_BAND_PLAIN)
+    return fail(b->lin->ctx, ATM_ERR_ARG, "atm_banded_set_kernel: unknown solve kernel");
+  if (!band_kernel_ok(b, kernel))
+    return fail(b->lin->ctx, ATM_ERR_UNSUPPORTED, "atm_banded_set_kernel: this kernel cannot solve this factorisation");
+  b->kernel = kernel;
+  return ATM_OK;
+}
+
+int atm_banded_autotune(atm_banded* b, double* x, const double* rhs, int reps, int32_t* kernel_out, float* us_out,
+                        int max) {
+  if (!b || !x || !rhs || reps < 1) return ATM_ERR_ARG;
+  atm_ctx* ctx = b->lin->ctx;
+  hipSetDevice(ctx->device);
+  hipStream_t st = stream_of(ctx);
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  HIPCHK(ctx, hipEventCreate(&e0));
+  hipError_t ee = hipEventCreate(&e1);
+  if (ee != hipSuccess) {
+    hipEventDestroy(e0);
+    return hipfail(ctx, ee, "hipEventCreate");
+  }
+  int best = -1, n = 0, rc = ATM_OK;
+  float best_us = 0.f;
+  for (int k = ATM_BAND_MFMA; k <= ATM_BAND_PLAIN && rc == ATM_OK; ++k) {
+    if (!band_kernel_ok(b, k)) continue;
+    rc = band_launch(b, k, x, rhs);  // warm-up
+    if (rc) break;
+    hipEventRecord(e0, st);
+    for (int i = 0; i < reps && rc == ATM_OK; ++i) rc = band_launch(b, k, x, rhs);
+    hipEventRecord(e1, st);
+    hipEventSynchronize(e1);
+    float ms = 0.f;
+    hipEventElapsedTime(&ms, e0, e1);
+    const float us = ms * 1e3f / reps;
+    if (kernel_out && n < max) kernel_out[n] = k;
+    if (us_out && n < max) us_out[n] = us;
+    ++n;
+    if (best < 0 || us < best_us) {
+      best = k;
+      best_us = us;
+    }
+  }
+  hipEventDestroy(e0);
+  hipEventDestroy(e1);
+  if (rc) return rc;
+  b->kernel = best;
+  return best;
 }
 
 int64_t atm_banded_nunique(const atm_banded* b) { return b ? b->nu : ATM_ERR_ARG; }

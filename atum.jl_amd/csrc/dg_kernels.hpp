@@ -123,9 +123,10 @@ ATM_DEV void surface_node(const DevDG& p, int Nq, int64_t e, const int idx[3], c
 }
 
 // Face-point-parallel surface term: the threads of an element deal out its face
-// points (faces of the directions in the mask), each writes faceMJ*nf*MJI of
-// its face point to buf[s * nfp + f], f = face*Nfpf + fp; surface_gather then
-// subtracts a node's slots in direction order, as the node-centric form does.
+// points (faces of the directions in the mask), each writes faceMJ*nf of its face
+// point to buf[s * nfp + f], f = face*Nfpf + fp; surface_gather then subtracts a
+// node's slots times its MJI in direction order, as the node-centric form does
+// (the same products, (faceMJ*nf)*MJI, in the same order).
 template <class L, int SF, bool FAST>
 ATM_DEV void surface_faces_lds(const DevDG& p, int Nq, int Np, int64_t e, int t, const double* q, double* buf) {
   constexpr int D = L::D, Ns = L::Ns, NA = L::NA;
@@ -164,14 +165,13 @@ ATM_DEV void surface_faces_lds(const DevDG& p, int Nq, int Np, int64_t e, int t,
     }
     if constexpr (FAST && has_fast_sflux<L, SF>()) fast_sflux<L, SF>(p.c, n, qm, am, qp, ap, nf);
     else dispatch_sflux<L, SF>(p, n, qm, am, qp, ap, nf);
-    const double MJI = p.MJI[node];
 #pragma unroll
-    for (int s = 0; s < Ns; ++s) buf[s * nfp + f] = fMJ * nf[s] * MJI;
+    for (int s = 0; s < Ns; ++s) buf[s * nfp + f] = fMJ * nf[s];
   }
 }
 
 template <class L>
-ATM_DEV void surface_gather(const DevDG& p, int Nq, const int idx[3], const double* buf, double* acc) {
+ATM_DEV void surface_gather(const DevDG& p, int Nq, const int idx[3], const double* buf, double* acc, double MJIn) {
   constexpr int D = L::D, Ns = L::Ns;
   const int Nfpf = p.Nfpf, nfp = 2 * D * Nfpf;
 #pragma unroll
@@ -185,7 +185,7 @@ ATM_DEV void surface_gather(const DevDG& p, int Nq, const int idx[3], const doub
     if (D == 3) fp = (r == 0) ? idx[1] + Nq * idx[2] : ((r == 1) ? idx[0] + Nq * idx[2] : idx[0] + Nq * idx[1]);
     const int f = (2 * r + side) * Nfpf + fp;
 #pragma unroll
-    for (int s = 0; s < Ns; ++s) acc[s] -= buf[s * nfp + f];
+    for (int s = 0; s < Ns; ++s) acc[s] -= buf[s * nfp + f] * MJIn;
   }
 }
 

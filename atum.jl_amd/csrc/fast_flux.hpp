@@ -9,11 +9,40 @@
 // h/c, so a differently rounded pressure would change the result far beyond
 // round-off of the flux.  Everything downstream is reformulated: the normal
 // physical flux is formed directly (F(q).n = [rho u.n, rho u (u.n) + p n,
-// (u.n)(rho e + p)]), reciprocals are shared, FMA contraction is allowed.
+// (u.n)(rho e + p)]), reciprocals are shared, FMA contraction is allowed, and the
+// reciprocals and square roots outside the pressures are the hardware estimates
+// refined by Newton steps (about one ulp) instead of the IEEE sequences.
 #pragma once
 #include "laws.hpp"
 
 namespace atm {
+
+// 1/x for positive normal x: the hardware reciprocal refined by two Newton
+// steps (within about one ulp), about a third of an IEEE division.
+__device__ __forceinline__ double fast_rcp(double x) {
+#pragma clang fp contract(fast)
+  double r = __builtin_amdgcn_rcp(x);
+  double e = fma(-x, r, 1.0);
+  r = fma(r, e, r);
+  e = fma(-x, r, 1.0);
+  return fma(r, e, r);
+}
+
+// sqrt(x) for positive normal x: the hardware reciprocal square root y refined by
+// two coupled Newton (Goldschmidt) steps on g = x*y ~ sqrt(x) and h = y/2 ~
+// 1/(2 sqrt(x)), then one residual correction g + h (x - g^2) (within about one ulp).
+__device__ __forceinline__ double fast_sqrt(double x) {
+#pragma clang fp contract(fast)
+  const double y = __builtin_amdgcn_rsq(x);
+  double g = x * y, h = 0.5 * y;
+#pragma unroll
+  for (int it = 0; it < 2; ++it) {
+    const double r = fma(-g, h, 0.5);
+    g = fma(g, r, g);
+    h = fma(h, r, h);
+  }
+  return fma(fma(-g, g, x), h, g);
+}
 
 template <class L, int SF>
 constexpr bool has_fast_sflux() {
@@ -41,7 +70,7 @@ ATM_DEV void fast_side(const LawConst& c, const double* q, const double* a, doub
   }
   {
 #pragma clang fp contract(fast)
-    S.ir = 1 / S.rho;
+    S.ir = fast_rcp(S.rho);
     S.un = 0;
 #pragma unroll
     for (int d = 0; d < D; ++d) {
@@ -92,16 +121,16 @@ ATM_DEV void fast_flux_combine(const LawConst& c, const double* n, const double*
   fast_normal_flux<D, GRAV>(c, M, qm, am, n, fm);
   fast_normal_flux<D, GRAV>(c, P, qp, ap, n, fpn);
   if constexpr (SF == ATM_FLUX_RUSANOV) {
-    const double wm = fabs(M.un) + sqrt(g * M.pf * M.ir);
-    const double wp = fabs(P.un) + sqrt(g * P.pf * P.ir);
+    const double wm = fabs(M.un) + fast_sqrt(g * M.pf * M.ir);
+    const double wp = fabs(P.un) + fast_sqrt(g * P.pf * P.ir);
     const double ws = 0.5 * fmax(wm, wp);
 #pragma unroll
     for (int s = 0; s < Ns; ++s) nf[s] = 0.5 * (fm[s] + fpn[s]) - ws * (qp[s] - qm[s]);
   } else {
     const double hm = (M.rhoe + M.p) * M.ir, hp = (P.rhoe + P.p) * P.ir;
-    const double cm = sqrt(g * M.p * M.ir), cp = sqrt(g * P.p * P.ir);
-    const double srm = sqrt(M.rho), srp = sqrt(P.rho);
-    const double iden = 1 / (srm + srp);
+    const double cm = fast_sqrt(g * M.p * M.ir), cp = fast_sqrt(g * P.p * P.ir);
+    const double srm = fast_sqrt(M.rho), srp = fast_sqrt(P.rho);
+    const double iden = fast_rcp(srm + srp);
     const double rho = srm * srp;
     double u[D], du[D];
     double un = 0, dun = 0, uu = 0, udu = 0;
@@ -117,7 +146,7 @@ ATM_DEV void fast_flux_combine(const LawConst& c, const double* n, const double*
     const double h = (srm * hm + srp * hp) * iden;
     const double cs = (srm * cm + srp * cp) * iden;
     const double drho = P.rho - M.rho, dp = P.p - M.p;
-    const double cm2 = 1 / (cs * cs);
+    const double cm2 = fast_rcp(cs * cs);
     const double w1 = 0.5 * fabs(un - cs) * (dp - rho * cs * dun) * cm2;
     const double w2 = 0.5 * fabs(un + cs) * (dp + rho * cs * dun) * cm2;
     const double w3 = fabs(un) * (drho - dp * cm2);

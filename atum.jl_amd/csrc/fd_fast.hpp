@@ -51,17 +51,6 @@ __device__ __forceinline__ double lm_series(double u) {
 
 constexpr double LM_U0 = 0.01;
 
-// 1/x for positive normal x: the hardware reciprocal refined by two Newton
-// steps (within about one ulp), about a third of an IEEE division.
-__device__ __forceinline__ double fast_rcp(double x) {
-#pragma clang fp contract(fast)
-  double r = __builtin_amdgcn_rcp(x);
-  double e = fma(-x, r, 1.0);
-  r = fma(r, e, r);
-  e = fma(-x, r, 1.0);
-  return fma(r, e, r);
-}
-
 // doubles per element of the LDS exchange buffer: pair shares (M*Ns per node)
 // and, afterwards, the face-point surface results (Ns per face point)
 template <class L, int NQ>
@@ -285,7 +274,7 @@ ATM_DEV void fd_fast_body(const DevDG& p, double* lds, int64_t blk, const double
     const double v = src[s] + MJIn * vol[s];
     acc[s] = assign ? v : alpha * dq0[s] + v;
   }
-  surface_gather<L>(p, Nq, idx, sx, acc);
+  surface_gather<L>(p, Nq, idx, sx, acc, MJIn);
   epilogue<L>(p, node, q1, acc, dq, qout, bdt);
 }
 

@@ -22,8 +22,8 @@ Resolved fast_variant() {
   Resolved r = make_resolved(launch_fd_fast<L, NQ, SF>, launch_fd_fast_batch<L, NQ, SF>, L::Ns,
                              fast_epw<L, NQ, SF>(), sizeof(double) * NQ * NQ, fast_lds_elem_bytes<L, NQ>(),
                              fast_max_threads<L, NQ, SF>());
-  // 2-D members of N = 3..7 with a compiled-in surface flux share one batched launch
-  if constexpr (L::D == 2 && SF >= 0 && NQ >= 4 && NQ <= 8) r.mbfn = launch_fd_fast_multi_batch<L, SF>;
+  // 2-D and 3-D members of N = 3..7 with a compiled-in surface flux share one batched launch
+  if constexpr (L::D >= 2 && SF >= 0 && NQ >= 4 && NQ <= 8) r.mbfn = launch_fd_fast_multi_batch<L, SF>;
   return r;
 }
 

@@ -176,6 +176,84 @@ def ensemble_workload(args, steps=None):
     }
 
 
+def ensemble3d_workload(args):
+    """3-D Euler-gravity bubbles (FAST EC + Roe, walls) of N = 3..7 with 2^3..5^3 elements
+    (PCG64 seed 20260104), LSRK54 with per-instance dt: all degrees in one launch per stage,
+    against one batch per degree (one launch per degree and stage) and each instance alone."""
+    import numpy as np
+    import atum_hip as A
+    import problems as P
+
+    rng = np.random.Generator(np.random.PCG64(20260104))
+    dgs, q0s, dts = [], [], []
+    for _ in range(args.instances3d):
+        N = int(rng.integers(3, 8))
+        Kd = int(rng.integers(2, 6))
+        v = P.julia_range(0.0, 2000.0, Kd + 1)
+        g = A.brickgrid(A.LobattoCell(N, 3), (v, v, v), (False,) * 3)
+        law = A.EulerGravityLaw(3, grav=9.81, problem=A.Problem(boundary="reflect"))
+        dgs.append(A.DGSEM(law=law, grid=g, volume_form=A.FluxDifferencingForm(A.EntropyConservativeFlux()),
+                           surface_numericalflux=A.RoeFlux(), precision="fast"))
+        q0s.append(P.rising_bubble(g.points))
+        dts.append((0.25 if N < 6 else 0.125) * (2000.0 / Kd) / N / 330.0)
+    ctx = dgs[0].ctx
+    dof = sum(q.size for q in q0s)
+    nsteps = args.steps
+    stages = 5 * nsteps
+
+    def timed(groups):
+        qs_all, ens_all = [], []
+        for idx in groups:
+            qs = [dgs[i].fieldarray(q0s[i]) for i in idx]
+            ens_all.append(A.LSRKEnsemble([dgs[i] for i in idx], qs, [dts[i] for i in idx]))
+            qs_all.append(qs)
+        for e in ens_all:
+            e.run(2)
+        ctx.sync()
+        ctx.record(0)
+        for e in ens_all:
+            e.run(nsteps)
+        ctx.record(1)
+        ms = ctx.elapsed_ms(0, 1)
+        ctx.sync()
+        out = {}
+        for idx, qs in zip(groups, qs_all):
+            for i, q in zip(idx, qs):
+                out[i] = q.numpy()
+        return ms, out, [e.batch.ngroups for e in ens_all]
+
+    ms_one, res_one, ng = timed([list(range(len(dgs)))])
+    by_deg = {}
+    for i, dg in enumerate(dgs):
+        by_deg.setdefault(dg.grid.N, []).append(i)
+    ms_deg, res_deg, _ = timed(list(by_deg.values()))
+    qs2 = [dg.fieldarray(q) for dg, q in zip(dgs, q0s)]
+    solvers = [A.LSRK54(dg, q, dt) for dg, q, dt in zip(dgs, qs2, dts)]
+    for s_, q in zip(solvers, qs2):
+        s_.run(q, 2)
+    ctx.sync()
+    ctx.record(2)
+    for s_, q in zip(solvers, qs2):
+        s_.run(q, nsteps)
+    ctx.record(3)
+    ms_alone = ctx.elapsed_ms(2, 3)
+    ctx.sync()
+    same = all(np.array_equal(res_one[i], res_deg[i]) and np.array_equal(res_one[i], q.numpy())
+               for i, q in enumerate(qs2))
+    if not same:
+        raise SystemExit("bench: batched, per-degree and alone states differ")
+    return {"metric": "aggregate FP64 DOF-updates/s per RK stage (batched 3-D ensemble)",
+            "value": dof * stages / (ms_one / 1e3), "unit": "DOF-updates/s", "n_gpus": 1, "steps": nsteps,
+            "warmup": 2, "ms_per_step": ms_one / nsteps, "higher_is_better": True, "scaling": "weak",
+            "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+            "config": {"workload": f"{len(dgs)} 3-D Euler-gravity bubbles, N~U{{3..7}}, K~U{{2..5}}^3, FAST EC + Roe, "
+                                   "LSRK54 (PCG64 seed 20260104)", "dof": dof, "kernel_groups": ng[0]},
+            "one_batch_per_degree": {"value": dof * stages / (ms_deg / 1e3), "ms_per_step": ms_deg / nsteps,
+                                     "launches_per_stage": len(by_deg)},
+            "alone": {"value": dof * stages / (ms_alone / 1e3), "ms_per_step": ms_alone / nsteps},
+            "bitwise_equal": True}
+
+
 def imex_workload(args):
     """32x32x10 hex elements on 20 km x 20 km x 10 km, N = 3, Euler-gravity on an isothermal
     background + 0.1 K seeded theta noise; ARK23 (split_rhs = false): explicit EC flux
@@ -213,6 +291,10 @@ def imex_workload(args):
     q = ex.fieldarray(q0)
     ark = A.ARK23(ex, lin, q, dt, split_rhs=False)
     ctx = ex.ctx
+    band_tuned = None
+    if args.tune:  # the banded solve kernel, timed on a right-hand side of the state's size
+        scratch = ex.fieldarray(q0)
+        band_tuned = ark.banded.autotune(ex.fieldarray(), scratch, reps=10)
     ark.run(q, 2)
     ctx.sync()
     ctx.record(0)
@@ -229,7 +311,9 @@ def imex_workload(args):
             "config": {"workload": "IMEX Euler-gravity 32x32x10, N=3, ARK23 unsplit, banded vertical solve",
                        "dof": dof, "dt": dt, "lines": 32 * 32 * 16, "precision": args.precision,
                        "elements_per_workgroup": {"explicit": ex.epw, "linear": lin.epw},
-                       "autotune_us_per_launch": {k: v[1] for k, v in tuned.items()} or None}}
+                       "autotune_us_per_launch": {k: v[1] for k, v in tuned.items()} or None,
+                       "banded_solve_kernel": ark.banded.kernel,
+                       "banded_autotune_us_per_solve": band_tuned[1] if band_tuned else None}}
 
 
 def weak_workload(args):
@@ -325,6 +409,7 @@ def serve_workload(args):
     reqs = make_stream(duration=args.serve_seconds)
     progress(f"serve: {len(reqs)} requests")
     server = A.Server(chunk=args.chunk, keep_results=False, imex_streams=args.imex_streams,
+                      max_active=args.serve_max_active,
                       tune=A.TuneCache() if args.serve_tune else None)
     stats = server.run(reqs)
     s = stats.summary()
@@ -355,6 +440,7 @@ def serve_workload(args):
             "config": {"workload": f"Poisson 50/s for {args.serve_seconds} s, mix 50/30/20 "
                                    "vortex/shallow-water/IMEX, steps U{50..500}", "requests": len(reqs),
                        "kinds": kinds, "chunk_steps": args.chunk, "imex_streams": args.imex_streams,
+                       "max_active": args.serve_max_active,
                        "autotune": bool(args.serve_tune)},
             "p50_s": s["p50_s"], "p99_s": s["p99_s"], "mean_s": s["mean_s"], "completed": s["completed"],
             "wall_s": s["wall_s"], "ticks": s["ticks"], "max_active": s["max_active"],
@@ -451,10 +537,12 @@ def main():
                          "reference: the reference's summation order, no FMA")
     ap.add_argument("--no-reference-order", action="store_true",
                     help="skip the extra timing of the reference-order kernel")
-    ap.add_argument("--workload", choices=["bubble", "ensemble", "imex", "vortex", "serve", "copy", "weak"],
+    ap.add_argument("--workload", choices=["bubble", "ensemble", "ensemble3d", "imex", "vortex", "serve", "copy",
+                                           "weak"],
                     default="bubble",
                     help="bubble (default, the metric's configuration); the others are extra lines")
     ap.add_argument("--instances", type=int, default=256)
+    ap.add_argument("--instances3d", type=int, default=64, help="ensemble3d: number of 3-D instances")
     ap.add_argument("--weak-K", type=int, default=256, help="weak workload: elements per direction")
     ap.add_argument("--serve-seconds", type=float, default=60.0,
                     help="serve: length of the Poisson arrival stream (the served configuration: 60 s)")
@@ -477,6 +565,8 @@ def main():
     ap.add_argument("--serve-tune", action="store_true",
                     help="serve: tune each new operator's workgroup shape on first use (under load; "
                          "measured slower on the default stream)")
+    ap.add_argument("--serve-max-active", type=int, default=512,
+                    help="serve: requests resident on the device at once (admission limit besides memory)")
     ap.add_argument("--imex-streams", type=int, default=3,
                     help="side streams for the IMEX requests of the served stream (0: the main stream)")
     args = ap.parse_args()
@@ -491,7 +581,8 @@ def main():
     import atum_hip as A
 
     if args.workload != "bubble":
-        fn = {"ensemble": ensemble_workload, "imex": imex_workload, "vortex": vortex_workload,
+        fn = {"ensemble": ensemble_workload, "ensemble3d": ensemble3d_workload, "imex": imex_workload,
+              "vortex": vortex_workload,
               "serve": serve_workload, "copy": copy_workload, "weak": weak_workload}[args.workload]
         print(json.dumps(fn(args)))
         return

@@ -249,6 +249,31 @@ int atm_banded_destroy(atm_banded* b);
 /* device bytes the factorisation holds (factors, block form, index tables).  No
  * counterpart in the reference. */
 int64_t atm_banded_device_bytes(const atm_banded* b);
+
+/* Solve kernels of atm_banded_solve (all give the solution of the same factors;
+ * the block forms agree with substitution to round-off):
+ *   MFMA   block form on the matrix cores, 16 lines per wave (Ns in {1,4,5}, Nq <= 6);
+ *   BLOCK  the same block form on the vector units (Ns in {1,4,5}, Nq <= 6);
+ *   REG    register-blocked substitution, 64 lines per wave (Ns in {1,4,5}, Nq <= 6);
+ *   LDS    substitution with the line matrix staged in LDS (when it fits);
+ *   PLAIN  substitution from global memory, one lane per line (always).
+ * AUTO takes the first available in that order.  atm_banded_set_kernel fails with
+ * ATM_ERR_UNSUPPORTED for a kernel that cannot solve this factorisation;
+ * atm_banded_get_kernel returns the kernel a solve would use.  atm_banded_autotune
+ * times `reps` solves x = A \ rhs with each available kernel (HIP events on the
+ * context's stream), keeps the fastest and returns it (per-kernel ids and
+ * microseconds per solve go to the optional arrays of capacity max); x is
+ * overwritten.  No counterpart in the reference (Bennu's ldiv! has one form). */
+#define ATM_BAND_AUTO (-1)
+#define ATM_BAND_MFMA 0
+#define ATM_BAND_BLOCK 1
+#define ATM_BAND_REG 2
+#define ATM_BAND_LDS 3
+#define ATM_BAND_PLAIN 4
+int atm_banded_set_kernel(atm_banded* b, int kernel);
+int atm_banded_get_kernel(const atm_banded* b);
+int atm_banded_autotune(atm_banded* b, double* x, const double* rhs, int reps, int32_t* kernel_out, float* us_out,
+                        int max);
 /* x = A^{-1} b for every line (ldiv!, src/odesolvers.jl:355); x may alias b. */
 int atm_banded_solve(atm_banded* b, double* x, const double* rhs);
 /* Copy the factored band to the host: lines x n x (2*bw+1) in [line][row][bw+col-row]. */

@@ -86,3 +86,33 @@ def test_batch_mixed_laws_and_modes():
         A.LSRK54(dg, alone, dt).run(alone, 5)
         dg.ctx.sync()
         assert np.array_equal(q.numpy(), alone.numpy())
+
+
+def _bubble_member(A, N, Kd, sflux="roe"):
+    v = P.julia_range(0, 2000, Kd + 1)
+    g = A.brickgrid(A.LobattoCell(N, 3), (v, v, v), (False,) * 3)
+    law = A.EulerGravityLaw(3, grav=9.81, problem=A.Problem(boundary="reflect"))
+    flux = A.RoeFlux() if sflux == "roe" else A.RusanovFlux()
+    dg = A.DGSEM(law=law, grid=g, volume_form=A.FluxDifferencingForm(A.EntropyConservativeFlux()),
+                 surface_numericalflux=flux, precision="fast")
+    return dg, P.rising_bubble(g.points), (0.25 if N < 6 else 0.125) * (2000 / Kd) / N / 330
+
+
+@pytest.mark.parametrize("sflux", ["roe", "rusanov"])
+def test_batch_fast_euler_gravity_3d_all_degrees_one_launch(sflux):
+    """3-D bubbles of N = 3..7 (FAST EC volume) in ONE launch per stage: the 3-D N = 6
+    member, which alone runs two elements in a 768-thread workgroup, takes one element
+    per workgroup here; the smaller members fill the launch's 512 threads; each member
+    ends bitwise where it would alone."""
+    import atum_hip as A
+    members = [_bubble_member(A, N, Kd, sflux) for N, Kd in ((3, 3), (4, 2), (5, 2), (6, 2), (7, 1))]
+    dgs = [m[0] for m in members]
+    qs = [dg.fieldarray(m[1]) for dg, m in zip(dgs, members)]
+    ens = A.LSRKEnsemble(dgs, qs, [m[2] for m in members])
+    assert ens.batch.ngroups == 1
+    ens.run(6)
+    for (dg, q0, dt), q in zip(members, qs):
+        alone = dg.fieldarray(q0)
+        A.LSRK54(dg, alone, dt).run(alone, 6)
+        dg.ctx.sync()
+        assert np.array_equal(q.numpy(), alone.numpy()), dg.grid.N
