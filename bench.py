@@ -565,7 +565,7 @@ def main():
     ap.add_argument("--serve-tune", action="store_true",
                     help="serve: tune each new operator's workgroup shape on first use (under load; "
                          "measured slower on the default stream)")
-    ap.add_argument("--serve-max-active", type=int, default=512,
+    ap.add_argument("--serve-max-active", type=int, default=64,
                     help="serve: requests resident on the device at once (admission limit besides memory)")
     ap.add_argument("--imex-streams", type=int, default=3,
                     help="side streams for the IMEX requests of the served stream (0: the main stream)")
