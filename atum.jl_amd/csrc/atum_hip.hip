@@ -847,47 +847,44 @@ band_solve_mfma_kernel(BandGeo g, const double* __restrict__ frags, const int32_
   };
 #pragma unroll
   for (int ks = 0; ks < KS; ++ks) v[ks] = 0.0;
-  // forward: L y = b, blocks it, it+1, it+2 in buffers 0, 1, 2, each refilled two blocks ahead
+  // forward: L y = b, blocks it, it+1, it+2 in buffers 0, 1, 2, each refilled two blocks ahead.
+  // The look-ahead loads are issued unconditionally (past the last block they reload the
+  // last block, whose values are not used): a load issued on one path only would make the
+  // compiler wait for every outstanding load, the look-ahead ones included, before each block.
+  auto fwd = [&](int ez) { return ez < KV ? ez : KV - 1; };
+  auto bwd = [&](int ez) { return ez >= 0 ? ez : 0; };
   fetch(0, 0, f0);
   fetch_rows(0, r0);
-  if (KV > 1) {
-    fetch(1, 0, f1);
-    fetch_rows(1, r1);
-  }
+  fetch(fwd(1), 0, f1);
+  fetch_rows(fwd(1), r1);
   for (int it = 0; it < KV; it += 3) {
-    if (it + 2 < KV) {
-      fetch(it + 2, 0, f2);
-      fetch_rows(it + 2, r2);
-    }
+    fetch(fwd(it + 2), 0, f2);
+    fetch_rows(fwd(it + 2), r2);
     block(it, it, f0, r0, 0);
     if (it + 1 >= KV) break;
-    if (it + 3 < KV) {
-      fetch(it + 3, 0, f0);
-      fetch_rows(it + 3, r0);
-    }
+    fetch(fwd(it + 3), 0, f0);
+    fetch_rows(fwd(it + 3), r0);
     block(it + 1, it + 1, f1, r1, 0);
     if (it + 2 >= KV) break;
-    if (it + 4 < KV) {
-      fetch(it + 4, 0, f1);
-      fetch_rows(it + 4, r1);
-    }
+    fetch(fwd(it + 4), 0, f1);
+    fetch_rows(fwd(it + 4), r1);
     block(it + 2, it + 2, f2, r2, 0);
   }
   __syncthreads();
   // backward: U x = y, rows from LDS, solution stored per block
   fetch(KV - 1, 2, f0);
-  if (KV > 1) fetch(KV - 2, 2, f1);
+  fetch(bwd(KV - 2), 2, f1);
   for (int it = 0; it < KV; it += 3) {
     const int ez = KV - 1 - it;
-    if (it + 2 < KV) fetch(ez - 2, 2, f2);
+    fetch(bwd(ez - 2), 2, f2);
     lds_rows(ez, r0);
     block(it, ez, f0, r0, 1);
     if (it + 1 >= KV) break;
-    if (it + 3 < KV) fetch(ez - 3, 2, f0);
+    fetch(bwd(ez - 3), 2, f0);
     lds_rows(ez - 1, r1);
     block(it + 1, ez - 1, f1, r1, 1);
     if (it + 2 >= KV) break;
-    if (it + 4 < KV) fetch(ez - 4, 2, f1);
+    fetch(bwd(ez - 4), 2, f1);
     lds_rows(ez - 2, r2);
     block(it + 2, ez - 2, f2, r2, 1);
   }

@@ -100,18 +100,21 @@ ATM_DEV void surface_node(const DevDG& p, int Nq, int64_t e, const int idx[3], c
     if (D == 2) fp = (r == 0) ? idx[1] : idx[0];
     if (D == 3) fp = (r == 0) ? idx[1] + Nq * idx[2] : ((r == 1) ? idx[0] + Nq * idx[2] : idx[0] + Nq * idx[1]);
     const int64_t jj = e * p.Nfp + (int64_t)face * p.Nfpf + fp;
+    // loads unconditional (a boundary face reads the element's first node as the
+    // neighbour and replaces it by the boundary state), so that none waits on a
+    // conditional path
+    const int tag = p.bctag[e * (2 * D) + face];
+    const int64_t idx_p = p.faceix_p[jj];
+    const int64_t idp = tag == 0 ? idx_p : e * p.Np;
     double n[D];
 #pragma unroll
     for (int d = 0; d < D; ++d) n[d] = p.normal[d * p.NF + jj];
     const double fMJ = p.faceMJ[jj];
-    const int tag = p.bctag[e * (2 * D) + face];
     double qp[Ns], ap[NA], nf[Ns];
-    if (tag == 0) {
-      const int64_t idp = p.faceix_p[jj];
 #pragma unroll
-      for (int s = 0; s < Ns; ++s) qp[s] = q[s * p.Ntot + idp];
-      load_aux<L>(p, idp, ap);
-    } else {
+    for (int s = 0; s < Ns; ++s) qp[s] = q[s * p.Ntot + idp];
+    load_aux<L>(p, idp, ap);
+    if (tag != 0) {
       boundarystate<L>(p.c, n, qn, tag, qp);
 #pragma unroll
       for (int s = 0; s < NA; ++s) ap[s] = an[s];
@@ -144,21 +147,23 @@ ATM_DEV void surface_faces_lds(const DevDG& p, int Nq, int Np, int64_t e, int t,
     id[r] = side ? Nq - 1 : 0;
     const int64_t node = e * Np + id[0] + Nq * (id[1] + Nq * id[2]);
     const int64_t jj = e * p.Nfp + f;
-    double qm[Ns], am[NA], n[D];
+    // every load first and unconditionally (a boundary face point reads its own node as
+    // the neighbour and replaces it by the boundary state below), so that none waits for
+    // another on a conditional path
+    const int tag = p.bctag[e * (2 * D) + face];
+    const int64_t idx_p = p.faceix_p[jj];
+    const int64_t idp = tag == 0 ? idx_p : node;
+    double qm[Ns], am[NA], n[D], qp[Ns], ap[NA], nf[Ns];
 #pragma unroll
     for (int s = 0; s < Ns; ++s) qm[s] = q[s * p.Ntot + node];
     load_aux<L>(p, node, am);
 #pragma unroll
     for (int d = 0; d < D; ++d) n[d] = p.normal[d * p.NF + jj];
     const double fMJ = p.faceMJ[jj];
-    const int tag = p.bctag[e * (2 * D) + face];
-    double qp[Ns], ap[NA], nf[Ns];
-    if (tag == 0) {
-      const int64_t idp = p.faceix_p[jj];
 #pragma unroll
-      for (int s = 0; s < Ns; ++s) qp[s] = q[s * p.Ntot + idp];
-      load_aux<L>(p, idp, ap);
-    } else {
+    for (int s = 0; s < Ns; ++s) qp[s] = q[s * p.Ntot + idp];
+    load_aux<L>(p, idp, ap);
+    if (tag != 0) {
       boundarystate<L>(p.c, n, qm, tag, qp);
 #pragma unroll
       for (int s = 0; s < NA; ++s) ap[s] = am[s];
@@ -239,30 +244,34 @@ ATM_DEV void fd_body(const DevDG& p, double* lds, int64_t blk, const double* __r
   for (int t = tid; t < Nq * Nq; t += blockDim.x) sD[t] = p.D[t];
   double q1[Ns], a1[NA], P1[NP];
   const int64_t node = e * Np + ijk;
-  double MJn = 0;
-  if (active) {
+  // every load first and unconditionally (a lane past the last element reads node 0 and
+  // discards it): state, aux, MJ, MJI, the metric rows of every direction and the stage's
+  // dq in flight together (one memory latency), then the primitives
+  const int64_t nl = active ? node : 0;
 #pragma unroll
-    for (int s = 0; s < Ns; ++s) q1[s] = q[s * p.Ntot + node];
-    load_aux<L>(p, node, a1);
-    PF::prim(p.c, q1, a1, P1);
-#pragma unroll
-    for (int k = 0; k < NP; ++k) sP[k * Np + ijk] = P1[k];
-    MJn = p.MJ[node];
-  }
-  // metric rows of every direction in the mask, loaded with the state (one latency)
+  for (int s = 0; s < Ns; ++s) q1[s] = q[s * p.Ntot + nl];
+  load_aux<L>(p, nl, a1);
+  const double MJn = p.MJ[nl];
+  const double MJIn = p.MJI[nl];
   double gall[D][D];
 #pragma unroll
   for (int r = 0; r < D; ++r)
 #pragma unroll
-    for (int d = 0; d < D; ++d)
-      gall[r][d] = (active && ((p.dirmask >> r) & 1)) ? p.g[(int64_t)(r + D * d) * p.Ntot + node] : 0.0;
-  int idx[3];
-  node_index<D>(ijk, Nq, idx);
-  const double MJIn = active ? p.MJI[node] : 0.0;
-  // the stage's dq, loaded with the state (its latency overlaps the volume term)
+    for (int d = 0; d < D; ++d) gall[r][d] = p.g[(int64_t)(r + D * d) * p.Ntot + nl];
   double dq0[Ns];
 #pragma unroll
-  for (int s = 0; s < Ns; ++s) dq0[s] = (active && !assign) ? dq[s * p.Ntot + node] : 0.0;
+  for (int s = 0; s < Ns; ++s) dq0[s] = 0.0;
+  if (!assign) {  // uniform over the launch
+#pragma unroll
+    for (int s = 0; s < Ns; ++s) dq0[s] = dq[s * p.Ntot + nl];
+  }
+  PF::prim(p.c, q1, a1, P1);
+  if (active) {
+#pragma unroll
+    for (int k = 0; k < NP; ++k) sP[k * Np + ijk] = P1[k];
+  }
+  int idx[3];
+  node_index<D>(ijk, Nq, idx);
   double acc[Ns];
   const int fdk = (NQ > 0) ? ATM_FD_PER_DIR : p.fdk;
   if (fdk == ATM_FD_NAIVE) {
@@ -432,20 +441,29 @@ ATM_DEV void weak_body(const DevDG& p, double* lds, int64_t blk, const double* _
   for (int t = tid; t < Nq * Nq; t += blockDim.x) sD[t] = p.D[t];
   double q1[Ns], a1[NA];
   const int64_t node = e * Np + ijk;
+  // every load first and unconditionally (a lane past the last element reads node 0 and
+  // discards it): state, aux, metric rows, MJ, MJI and the stage's dq in flight together
+  const int64_t nl = active ? node : 0;
+#pragma unroll
+  for (int s = 0; s < Ns; ++s) q1[s] = q[s * p.Ntot + nl];
+  load_aux<L>(p, nl, a1);
+  double gl[D][D];
+#pragma unroll
+  for (int r = 0; r < D; ++r)
+#pragma unroll
+    for (int dd = 0; dd < D; ++dd) gl[r][dd] = p.g[(int64_t)(r + D * dd) * p.Ntot + nl];
+  const double MJn = p.MJ[nl];
+  const double MJIn = p.MJI[nl];
+  double dq0[Ns];
+#pragma unroll
+  for (int s = 0; s < Ns; ++s) dq0[s] = 0.0;
+  if (!assign) {  // uniform over the launch
+#pragma unroll
+    for (int s = 0; s < Ns; ++s) dq0[s] = dq[s * p.Ntot + nl];
+  }
   if (active) {
-#pragma unroll
-    for (int s = 0; s < Ns; ++s) q1[s] = q[s * p.Ntot + node];
-    load_aux<L>(p, node, a1);
-    // metric rows of the directions in the mask, loaded once (not per state)
-    double gl[D][D];
-#pragma unroll
-    for (int r = 0; r < D; ++r)
-#pragma unroll
-      for (int dd = 0; dd < D; ++dd)
-        gl[r][dd] = ((p.dirmask >> r) & 1) ? p.g[(int64_t)(r + D * dd) * p.Ntot + node] : 0.0;
     double F[D][Ns];
     L::flux(p.c, q1, a1, F);
-    const double MJn = p.MJ[node];
 #pragma unroll
     for (int s = 0; s < Ns; ++s) {
       int slot = 0;
@@ -461,11 +479,6 @@ ATM_DEV void weak_body(const DevDG& p, double* lds, int64_t blk, const double* _
       }
     }
   }
-  // epilogue operands loaded before the barrier (their latency overlaps the volume term)
-  double dq0[Ns];
-  const double MJIn = active ? p.MJI[node] : 0.0;
-#pragma unroll
-  for (int s = 0; s < Ns; ++s) dq0[s] = (active && !assign) ? dq[s * p.Ntot + node] : 0.0;
   __syncthreads();
   if (!active) return;
   int idx[3];

@@ -174,27 +174,34 @@ ATM_DEV void fd_fast_body(const DevDG& p, double* lds, int64_t blk, const double
   for (int t = tid; t < Nq * Nq; t += blockDim.x) sD[t] = p.D[t];
   double q1[Ns], a1[NA], P1[NP];
   const int64_t node = e * Np + ijk;
-  double MJn = 0;
-  if (active) {
+  // Every load of the node's data first and unconditionally (a lane past the last element
+  // reads node 0 and discards it): a load on a conditional path would make the compiler
+  // wait for it before the next ones, so state, aux, MJ, MJI, the stage's dq and the metric
+  // rows of every direction are in flight together (one memory latency); the primitives
+  // follow.
+  const int64_t nl = active ? node : 0;
 #pragma unroll
-    for (int s = 0; s < Ns; ++s) q1[s] = q[s * p.Ntot + node];
-    load_aux<L>(p, node, a1);
-    FE::prim(p.c, q1, a1, P1);
-#pragma unroll
-    for (int k = 0; k < NP; ++k) sP[k * Np + ijk] = P1[k];
-    MJn = p.MJ[node];
-  }
-  // epilogue operands loaded with the state (their latency overlaps the volume term)
+  for (int s = 0; s < Ns; ++s) q1[s] = q[s * p.Ntot + nl];
+  load_aux<L>(p, nl, a1);
+  const double MJn = p.MJ[nl];
+  const double MJIn = p.MJI[nl];
   double dq0[Ns];
-  const double MJIn = active ? p.MJI[node] : 0.0;
 #pragma unroll
-  for (int s = 0; s < Ns; ++s) dq0[s] = (active && !assign) ? dq[s * p.Ntot + node] : 0.0;
-  // metric rows of every direction, loaded with the state (one memory latency)
+  for (int s = 0; s < Ns; ++s) dq0[s] = 0.0;
+  if (!assign) {  // uniform over the launch
+#pragma unroll
+    for (int s = 0; s < Ns; ++s) dq0[s] = dq[s * p.Ntot + nl];
+  }
   double gall[D][D];
 #pragma unroll
   for (int r = 0; r < D; ++r)
 #pragma unroll
-    for (int d = 0; d < D; ++d) gall[r][d] = active ? p.g[(int64_t)(r + D * d) * p.Ntot + node] : 0.0;
+    for (int d = 0; d < D; ++d) gall[r][d] = p.g[(int64_t)(r + D * d) * p.Ntot + nl];
+  FE::prim(p.c, q1, a1, P1);
+  if (active) {
+#pragma unroll
+    for (int k = 0; k < NP; ++k) sP[k * Np + ijk] = P1[k];
+  }
   int idx[3];
   node_index<D>(ijk, Nq, idx);
   const double inv_gm1 = 1 / (p.c.gamma - 1);
