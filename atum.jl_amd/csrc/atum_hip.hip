@@ -61,7 +61,7 @@ struct atm_banded {
   // [L_{i,i-1} | L_ii^-1 | U_{i,i+1} | U_ii^-1], each BS x BS row-major (null: not built)
   double* blocks = nullptr;
   double* frags = nullptr;        // the block form as MFMA A-operand fragments (band_frags_kernel)
-  int32_t* perm16 = nullptr;      // as perm, runs padded to 16 lines
+  int32_t* perm16 = nullptr;      // as perm, runs of 16 lines; padding repeats the run's first line
   int32_t* wave_mat16 = nullptr;
   int64_t nwaves16 = 0;
   int32_t* elem_col = nullptr;  // column of each element (-1: none)
@@ -597,9 +597,7 @@ band_solve_blk_kernel(BandGeo g, const double* __restrict__ blocks, const int32_
   __shared__ double mats[2][FB];
   __shared__ double vec[LPW][BS + 1];
   const int lane = threadIdx.x, j = lane / PARTS, part = lane % PARTS;
-  const int32_t pl = perm[blockIdx.x * (int64_t)LPW + j];
-  const bool active = pl >= 0;
-  const int64_t line = active ? pl : 0;  // padding lanes read line 0 and store nothing
+  const int64_t line = perm[blockIdx.x * (int64_t)LPW + j];  // padding lines repeat a line of the run
   const int KV = g.KV;
   const double* Bu = blocks + (int64_t)wave_mat[blockIdx.x] * KV * 2 * FB;  // block i: [fwd FB | bwd FB]
   const int64_t c = line / g.Nh;
@@ -683,7 +681,7 @@ band_solve_blk_kernel(BandGeo g, const double* __restrict__ blocks, const int32_
         const int m = part * RPL + k;
         if (m < BS) {
           vec[j][m] = t[k];
-          if (active) x[gi(ez, m)] = t[k];
+          x[gi(ez, m)] = t[k];
         }
       }
       if (it + 1 < KV) stage(1 - buf);
@@ -695,218 +693,241 @@ band_solve_blk_kernel(BandGeo g, const double* __restrict__ blocks, const int32_
   }
 }
 
-// Fragments of the block form for v_mfma_f64_16x16x4f64, per matrix u, element
-// block i and matrix mi (0: -L_{i,i-1}, 1: L_ii^-1, 2: -U_{i,i+1}, 3: U_ii^-1):
-// fragment (ti, ks) holds, for lane l, M[16 ti + (l & 15)][4 ks + (l >> 4)] (zero
-// outside BS x BS), i.e. the A operand of one MFMA, 64 doubles in lane order.
-__global__ void band_frags_kernel(BandGeo g, const double* __restrict__ blocks, double* __restrict__ frags, int TI,
-                                  int KS) {
+// Fragments of the block form for v_mfma_f64_4x4x4_4b_f64, per matrix u, element
+// block i and matrix mi (0: -L_{i,i-1}, 1: L_ii^-1, 2: -U_{i,i+1}, 3: U_ii^-1), in
+// 4 x 4 tiles (ti, tk): 16 doubles, element (r, c) of the tile at r + 4 c, i.e.
+// M[4 ti + r][4 tk + c] (zero outside BS x BS).  Lane l takes element
+// (l & 3, l >> 4) as its A operand, the same for the instruction's four blocks.
+__global__ void band_frags_kernel(BandGeo g, const double* __restrict__ blocks, double* __restrict__ frags, int KT) {
   const int64_t ui = blockIdx.x;
-  const int BS = g.bw + 1, BB = BS * BS, NF = TI * KS;
+  const int BS = g.bw + 1, BB = BS * BS, NT = KT * KT;
   const double* B = blocks + ui * 4 * (int64_t)BB;
-  double* F = frags + ui * 4 * (int64_t)NF * 64;
-  for (int t = threadIdx.x; t < 4 * NF * 64; t += blockDim.x) {
-    const int l = t % 64, f = (t / 64) % NF, mi = t / (64 * NF);
-    const int row = 16 * (f / KS) + (l & 15), colk = 4 * (f % KS) + (l >> 4);
-    double v = (row < BS && colk < BS) ? B[mi * BB + row * BS + colk] : 0.0;
+  double* F = frags + ui * 4 * (int64_t)NT * 16;
+  for (int t = threadIdx.x; t < 4 * NT * 16; t += blockDim.x) {
+    const int el = t % 16, tile = (t / 16) % NT, mi = t / (16 * NT);
+    const int row = 4 * (tile / KT) + (el & 3), col = 4 * (tile % KT) + (el >> 2);
+    const double v = (row < BS && col < BS) ? B[mi * BB + row * BS + col] : 0.0;
     F[t] = (mi == 0 || mi == 2) ? -v : v;
   }
 }
 
-typedef double atm_v4d __attribute__((ext_vector_type(4)));
+// Banded solve in block form on the matrix cores with v_mfma_f64_4x4x4_4b_f64.  A
+// workgroup of W waves takes 16 W lines of one matrix; each wave takes 16 of them and
+// the instruction's four blocks are four groups of four lines (line lw = lane & 15 of
+// the wave).  For block b and lanes l = j + 4 b + 16 q the instruction takes A[i][k]
+// from lane i + 4 b + 16 k, B[k][j] from lane j + 4 b + 16 k and returns D[i][j] in
+// lane j + 4 b + 16 i, so the line vector is held one double per 4-row tile (lane: row
+// 4 tk + (lane >> 4) of line lw) and a product's result tile ti is at once the B
+// operand of tile tk = ti of the next product.  Per block of rows
+//   t = r - C v  (C negated in the fragments, r as accumulator),  v = D^-1 t,
+// one instruction per nonzero 4 x 4 tile (the coupling blocks are strictly
+// triangular and the diagonal inverses triangular, so about half the tiles are
+// skipped), the row tiles of a product interleaved as independent accumulator
+// chains.  The 4 x 4 tiles waste no matrix-core work on padding rows and each
+// instruction occupies the matrix core 16 cycles.
+//
+// The operator fragments of a block (the two matrices of the pass, 2 NT tiles of 16
+// doubles) are loaded once per workgroup, each thread a few 16-byte pieces, two
+// blocks ahead into registers, written to one of two LDS buffers a block ahead and
+// read from there by every wave (four lanes share each element).  The right-hand-side
+// rows of the forward pass are loaded two blocks ahead into registers; the forward
+// solution is kept in LDS (yb[r * (16 W + 1) + line]); the backward pass reads its
+// rows from there and stores each finished block at once.  The block loops run whole
+// rounds of three blocks with the computation of blocks past the column's end
+// skipped (the look-ahead loads are clamped to the last block and issued
+// unconditionally, and no loop exit sits between a load and its use): every register
+// buffer stays in place across iterations, so the compiler never waits for all
+// outstanding loads.  Padding lanes of a run repeat the run's first line: they
+// compute the same values and store them to the same place, so every lane stores.
+template <int NS, int NQ, int W>
+struct BandMfmaShape {
+  static constexpr int BS = NS * NQ, KT = (BS + 3) / 4, NT = KT * KT, LDY = 16 * W + 1;
+  static constexpr int FSET = 2 * NT * 16;                                           // doubles of a pass's fragments
+  static constexpr int FPAD = (FSET + 128 * W - 1) / (128 * W) * (128 * W), CH = FPAD / (128 * W);  // 16-byte pieces per thread
+};
 
-// Banded solve in block form on the matrix cores: a wave takes 16 lines (one
-// matrix per wave), line j = lane & 15.  Per block of rows the line vector is
-// held in B-operand order (lane holds rows 4 ks + (lane >> 4)), which is also
-// the order of the MFMA result rows (16 ti + (lane >> 4) + 4 r), so the two
-// products
-//   t = r - C v  (C negated in the fragments, r as accumulator),  v = D^-1 t
-// run back to back in registers.  Forward pass: right-hand-side rows and the
-// (L2-resident) operator fragments are loaded two blocks ahead into fixed
-// register buffers; the forward solution is kept in LDS (yb[r * 17 + j]).
-// Backward pass: rows come from LDS, each finished block is stored at once.
-template <int NS, int NQ>
-__global__ void __launch_bounds__(64)
+typedef double atm_v2d __attribute__((ext_vector_type(2)));
+
+template <int NS, int NQ, int W>
+__global__ void __launch_bounds__(64 * W)
 band_solve_mfma_kernel(BandGeo g, const double* __restrict__ frags, const int32_t* __restrict__ perm,
-                       const int32_t* __restrict__ wave_mat, double* x, const double* b) {
-  constexpr int BS = NS * NQ, TI = (BS + 15) / 16, KS = (BS + 3) / 4, NF = TI * KS, LD = 17;
-  constexpr bool FULL = BS % 4 == 0;  // every lane's rows 4 ks + qq lie inside the block
-  extern __shared__ double yb[];
-  const int l = threadIdx.x, j = l & 15, qq = l >> 4;
-  const int32_t pl = perm[blockIdx.x * (int64_t)16 + j];
-  const bool active = pl >= 0;
-  const int64_t line = active ? pl : 0;  // padding lanes read line 0 and store nothing
+                       const int32_t* __restrict__ run_mat, double* x, const double* b) {
+  using S = BandMfmaShape<NS, NQ, W>;
+  constexpr int BS = S::BS, KT = S::KT, NT = S::NT, LDY = S::LDY, FPAD = S::FPAD, CH = S::CH;
+  constexpr bool FULL = BS % 4 == 0;  // every lane's rows 4 tk + q lie inside the block
+  constexpr int LA = 2, NBUF = LA + 1;
+  extern __shared__ double lds[];
   const int KV = g.KV, n = g.n;
-  const double* Fu = frags + (int64_t)wave_mat[blockIdx.x] * KV * 4 * NF * 64;
+  double* yb = lds;                                                 // n x LDY
+  double* fbuf = lds + (size_t)n * LDY;                             // 2 x FPAD
+  int32_t* cl = reinterpret_cast<int32_t*>(fbuf + 2 * FPAD);        // 16 W x KV
+  const int tid = threadIdx.x, l = tid & 63, lw = l & 15, q = l >> 4, fe = (l & 3) + 4 * q;
+  const int lg = 16 * (tid >> 6) + lw;  // line of the lane in the workgroup
+  const int64_t line = perm[blockIdx.x * (int64_t)(16 * W) + lg];
+  const double* Fu = frags + (int64_t)run_mat[blockIdx.x] * KV * 4 * NT * 16;
   const int64_t c = line / g.Nh;
   const int h = (int)(line - c * g.Nh);
-  // the elements of each line's column, staged in LDS: cl[j * KV + ez]
-  int32_t* cl = reinterpret_cast<int32_t*>(yb + (size_t)n * LD);
-  for (int ez = qq; ez < KV; ez += 4) cl[j * KV + ez] = g.colelem[c * KV + ez];
-  __syncthreads();
+  for (int ez = q; ez < KV; ez += 4) cl[lg * KV + ez] = g.colelem[c * KV + ez];
   auto gaddr = [&](int ez, int m) -> int64_t {
-    return (int64_t)(m % NS) * g.Ntot + h + (int64_t)g.Nh * (m / NS) + (int64_t)g.Np * cl[j * KV + ez];
+    return (int64_t)(m % NS) * g.Ntot + h + (int64_t)g.Nh * (m / NS) + (int64_t)g.Np * cl[lg * KV + ez];
   };
   auto row_ok = [&](int m) -> bool { return FULL || m < BS; };
-  double v[KS];
-  double f0[2 * NF], f1[2 * NF], f2[2 * NF];  // fragments of three consecutive blocks (fixed buffers)
-  double r0[KS], r1[KS], r2[KS];              // right-hand-side rows of the same blocks (forward pass)
-  // Tiles (16 rows x 4 columns) that are identically zero: with BS = bw + 1 the
-  // coupling blocks are strictly triangular (L_{i,i-1} strictly upper, U_{i,i+1}
-  // strictly lower) and the diagonal inverses triangular (L_ii^-1 lower, U_ii^-1
-  // upper), so some tiles hold only exact zeros; their MFMAs would add +0 and are
-  // skipped together with their fragment loads.  mi: 0 -L_{i,i-1}, 1 L_ii^-1,
-  // 2 -U_{i,i+1}, 3 U_ii^-1.
-  auto tile_used = [](int mi, int ti, int ks) -> bool {
-    const int r0 = 16 * ti, r1 = (16 * ti + 15 < BS - 1) ? 16 * ti + 15 : BS - 1;
-    const int c0 = 4 * ks, c1 = (4 * ks + 3 < BS - 1) ? 4 * ks + 3 : BS - 1;
+  // 4 x 4 tiles that hold only exact zeros (their instructions would add +0; skipped with
+  // their operand reads): L_{i,i-1} is strictly upper, L_ii^-1 lower, U_{i,i+1} strictly
+  // lower, U_ii^-1 upper triangular.
+  auto tile_used = [](int mi, int ti, int tk) -> bool {
+    const int r0 = 4 * ti, r1 = (4 * ti + 3 < BS - 1) ? 4 * ti + 3 : BS - 1;
+    const int c0 = 4 * tk, c1 = (4 * tk + 3 < BS - 1) ? 4 * tk + 3 : BS - 1;
     if (mi == 0) return r0 < c1;   // some r < c
     if (mi == 1) return r1 >= c0;  // some r >= c
     if (mi == 2) return r1 > c0;   // some r > c
     return r0 <= c1;               // some r <= c
   };
-  auto fetch = [&](int ez, int m0, double* dst) {
+  // this thread's pieces of a block's fragments (the allocation is padded by FPAD doubles,
+  // so the pieces past the last block's set stay inside it)
+  auto fetch_share = [&](int ez, int pass, atm_v2d* dst) {
+    const atm_v2d* src = reinterpret_cast<const atm_v2d*>(Fu + (int64_t)(ez * 4 + 2 * pass) * NT * 16);
 #pragma unroll
-    for (int f = 0; f < 2 * NF; ++f)
-      if (tile_used(m0 + f / NF, (f % NF) / KS, (f % NF) % KS))
-        dst[f] = Fu[((int64_t)(ez * 4 + m0) * NF + f) * 64 + l];
+    for (int k = 0; k < CH; ++k) dst[k] = src[tid + 64 * W * k];
+  };
+  auto put_share = [&](int buf, const atm_v2d* src) {
+    atm_v2d* d = reinterpret_cast<atm_v2d*>(fbuf + buf * FPAD);
+#pragma unroll
+    for (int k = 0; k < CH; ++k) d[tid + 64 * W * k] = src[k];
   };
   auto fetch_rows = [&](int ez, double* dst) {
 #pragma unroll
-    for (int ks = 0; ks < KS; ++ks) {
-      const int m = 4 * ks + qq;
-      dst[ks] = row_ok(m) ? b[gaddr(ez, m)] : 0.0;
+    for (int tk = 0; tk < KT; ++tk) {
+      const int m = 4 * tk + q;
+      dst[tk] = row_ok(m) ? b[gaddr(ez, m)] : 0.0;
     }
   };
   auto lds_rows = [&](int ez, double* dst) {
 #pragma unroll
-    for (int ks = 0; ks < KS; ++ks) {
-      const int m = 4 * ks + qq;
-      dst[ks] = row_ok(m) ? yb[(ez * BS + m) * LD + j] : 0.0;
+    for (int tk = 0; tk < KT; ++tk) {
+      const int m = 4 * tk + q;
+      dst[tk] = row_ok(m) ? yb[(ez * BS + m) * LDY + lg] : 0.0;
     }
   };
-  // one block of rows: t = r - C v (it > 0), v = D^-1 t.  Each tile's k-sum is
-  // split over two accumulators (even and odd k steps) for more independent
-  // MFMA chains.
+  double v[KT];
+  // one block of rows: t = r - C v (it > 0), v = D^-1 t; fragments from LDS buffer fa
   auto block = [&](int it, int ez, const double* fa, const double* rr, int pass) {
-    double t[KS];
+    double a[2 * NT];
+#pragma unroll
+    for (int f = 0; f < 2 * NT; ++f)
+      if (tile_used(2 * pass + f / NT, (f % NT) / KT, (f % NT) % KT)) a[f] = fa[f * 16 + fe];
+    double t[KT];
+#pragma unroll
+    for (int ti = 0; ti < KT; ++ti) t[ti] = rr[ti];
     if (it > 0) {
-      atm_v4d a0[TI], a1[TI];
 #pragma unroll
-      for (int ti = 0; ti < TI; ++ti)
+      for (int tk = 0; tk < KT; ++tk)
 #pragma unroll
-        for (int q = 0; q < 4; ++q) {
-          a0[ti][q] = (4 * ti + q < KS) ? rr[(4 * ti + q) < KS ? 4 * ti + q : 0] : 0.0;
-          a1[ti][q] = 0.0;
-        }
-#pragma unroll
-      for (int ks = 0; ks < KS; ++ks)
-#pragma unroll
-        for (int ti = 0; ti < TI; ++ti) {
-          if (!tile_used(2 * pass, ti, ks)) continue;
-          if (ks % 2 == 0) a0[ti] = __builtin_amdgcn_mfma_f64_16x16x4f64(fa[ti * KS + ks], v[ks], a0[ti], 0, 0, 0);
-          else a1[ti] = __builtin_amdgcn_mfma_f64_16x16x4f64(fa[ti * KS + ks], v[ks], a1[ti], 0, 0, 0);
-        }
-#pragma unroll
-      for (int ti = 0; ti < TI; ++ti)
-#pragma unroll
-        for (int q = 0; q < 4; ++q)
-          if (4 * ti + q < KS) t[4 * ti + q] = a0[ti][q] + a1[ti][q];
-    } else {
-#pragma unroll
-      for (int ks = 0; ks < KS; ++ks) t[ks] = rr[ks];
-    }
-    {
-      atm_v4d a0[TI], a1[TI];
-#pragma unroll
-      for (int ti = 0; ti < TI; ++ti)
-#pragma unroll
-        for (int q = 0; q < 4; ++q) a0[ti][q] = a1[ti][q] = 0.0;
-#pragma unroll
-      for (int ks = 0; ks < KS; ++ks)
-#pragma unroll
-        for (int ti = 0; ti < TI; ++ti) {
-          if (!tile_used(2 * pass + 1, ti, ks)) continue;
-          if (ks % 2 == 0) a0[ti] = __builtin_amdgcn_mfma_f64_16x16x4f64(fa[NF + ti * KS + ks], t[ks], a0[ti], 0, 0, 0);
-          else a1[ti] = __builtin_amdgcn_mfma_f64_16x16x4f64(fa[NF + ti * KS + ks], t[ks], a1[ti], 0, 0, 0);
-        }
-#pragma unroll
-      for (int ti = 0; ti < TI; ++ti)
-#pragma unroll
-        for (int q = 0; q < 4; ++q)
-          if (4 * ti + q < KS) v[4 * ti + q] = a0[ti][q] + a1[ti][q];
+        for (int ti = 0; ti < KT; ++ti)
+          if (tile_used(2 * pass, ti, tk))
+            t[ti] = __builtin_amdgcn_mfma_f64_4x4x4f64(a[ti * KT + tk], v[tk], t[ti], 0, 0, 0);
     }
 #pragma unroll
-    for (int ks = 0; ks < KS; ++ks) {
-      const int m = 4 * ks + qq;
+    for (int ti = 0; ti < KT; ++ti) v[ti] = 0.0;
+#pragma unroll
+    for (int tk = 0; tk < KT; ++tk)
+#pragma unroll
+      for (int ti = 0; ti < KT; ++ti)
+        if (tile_used(2 * pass + 1, ti, tk))
+          v[ti] = __builtin_amdgcn_mfma_f64_4x4x4f64(a[NT + ti * KT + tk], t[tk], v[ti], 0, 0, 0);
+#pragma unroll
+    for (int ti = 0; ti < KT; ++ti) {
+      const int m = 4 * ti + q;
       if (row_ok(m)) {
-        if (pass == 0) yb[(ez * BS + m) * LD + j] = v[ks];
-        else if (active) x[gaddr(ez, m)] = v[ks];
+        if (pass == 0) yb[(ez * BS + m) * LDY + lg] = v[ti];
+        else x[gaddr(ez, m)] = v[ti];
       }
     }
   };
-#pragma unroll
-  for (int ks = 0; ks < KS; ++ks) v[ks] = 0.0;
-  // forward: L y = b, blocks it, it+1, it+2 in buffers 0, 1, 2, each refilled two blocks ahead.
-  // The look-ahead loads are issued unconditionally (past the last block they reload the
-  // last block, whose values are not used): a load issued on one path only would make the
-  // compiler wait for every outstanding load, the look-ahead ones included, before each block.
+  atm_v2d fr[NBUF][CH];   // fragment pieces of the blocks in flight
+  double rbuf[NBUF][KT];  // right-hand-side rows of the forward pass in flight
   auto fwd = [&](int ez) { return ez < KV ? ez : KV - 1; };
   auto bwd = [&](int ez) { return ez >= 0 ? ez : 0; };
-  fetch(0, 0, f0);
-  fetch_rows(0, r0);
-  fetch(fwd(1), 0, f1);
-  fetch_rows(fwd(1), r1);
-  for (int it = 0; it < KV; it += 3) {
-    fetch(fwd(it + 2), 0, f2);
-    fetch_rows(fwd(it + 2), r2);
-    block(it, it, f0, r0, 0);
-    if (it + 1 >= KV) break;
-    fetch(fwd(it + 3), 0, f0);
-    fetch_rows(fwd(it + 3), r0);
-    block(it + 1, it + 1, f1, r1, 0);
-    if (it + 2 >= KV) break;
-    fetch(fwd(it + 4), 0, f1);
-    fetch_rows(fwd(it + 4), r1);
-    block(it + 2, it + 2, f2, r2, 0);
+#pragma unroll
+  for (int tk = 0; tk < KT; ++tk) v[tk] = 0.0;
+  __syncthreads();  // cl
+  // forward: L y = b.  Round it: blocks it + u (u < NBUF) in register buffers u.  Before
+  // block i is computed, the loads of block i + LA are issued into the buffer of block i - 1
+  // and the fragments of block i + 1 (loaded a block ago) are written to LDS buffer
+  // (i + 1) & 1, which block i - 1 finished reading before the last barrier.
+#pragma unroll
+  for (int u = 0; u < LA; ++u) {
+    fetch_share(fwd(u), 0, fr[u]);
+    fetch_rows(fwd(u), rbuf[u]);
   }
+  put_share(0, fr[0]);
   __syncthreads();
+  for (int it = 0; it < KV; it += NBUF) {
+#pragma unroll
+    for (int u = 0; u < NBUF; ++u) {
+      const int i = it + u;
+      fetch_share(fwd(i + LA), 0, fr[(u + LA) % NBUF]);
+      fetch_rows(fwd(i + LA), rbuf[(u + LA) % NBUF]);
+      put_share((i + 1) & 1, fr[(u + 1) % NBUF]);
+      if (i < KV) block(i, i, fbuf + (i & 1) * FPAD, rbuf[u], 0);
+      __syncthreads();
+    }
+  }
   // backward: U x = y, rows from LDS, solution stored per block
-  fetch(KV - 1, 2, f0);
-  fetch(bwd(KV - 2), 2, f1);
-  for (int it = 0; it < KV; it += 3) {
-    const int ez = KV - 1 - it;
-    fetch(bwd(ez - 2), 2, f2);
-    lds_rows(ez, r0);
-    block(it, ez, f0, r0, 1);
-    if (it + 1 >= KV) break;
-    fetch(bwd(ez - 3), 2, f0);
-    lds_rows(ez - 1, r1);
-    block(it + 1, ez - 1, f1, r1, 1);
-    if (it + 2 >= KV) break;
-    fetch(bwd(ez - 4), 2, f1);
-    lds_rows(ez - 2, r2);
-    block(it + 2, ez - 2, f2, r2, 1);
+#pragma unroll
+  for (int u = 0; u < LA; ++u) fetch_share(bwd(KV - 1 - u), 1, fr[u]);
+  put_share(0, fr[0]);
+  __syncthreads();
+  for (int it = 0; it < KV; it += NBUF) {
+#pragma unroll
+    for (int u = 0; u < NBUF; ++u) {
+      const int i = it + u, ez = KV - 1 - i;
+      fetch_share(bwd(ez - LA), 1, fr[(u + LA) % NBUF]);
+      put_share((i + 1) & 1, fr[(u + 1) % NBUF]);
+      if (ez >= 0) {
+        lds_rows(ez, rbuf[u]);
+        block(i, ez, fbuf + (i & 1) * FPAD, rbuf[u], 1);
+      }
+      __syncthreads();
+    }
   }
 }
 
-size_t band_mfma_lds_bytes(int n, int KV) { return (size_t)n * 17 * sizeof(double) + (size_t)16 * KV * sizeof(int32_t); }
+// waves per workgroup of the matrix-core solve: 4 when the workgroup's LDS fits, else 1
+size_t band_mfma_lds_bytes(int n, int KV, int BS, int W) {
+  const int KT = (BS + 3) / 4, NT = KT * KT, FSET = 2 * NT * 16;
+  const int FPAD = (FSET + 128 * W - 1) / (128 * W) * (128 * W);
+  return (size_t)n * (16 * W + 1) * sizeof(double) + (size_t)2 * FPAD * sizeof(double) +
+         (size_t)16 * W * KV * sizeof(int32_t);
+}
+
+// doubles of padding after the fragments: the largest FPAD - FSET over the shapes
+size_t band_mfma_frag_pad(int BS) {
+  const int KT = (BS + 3) / 4;
+  return (size_t)2 * KT * KT * 16 + 128;
+}
 
 typedef void (*BandSolveMfmaFn)(BandGeo, const double*, const int32_t*, const int32_t*, double*, const double*);
 
-BandSolveMfmaFn band_mfma_kernel(int Ns, int Nq) {
-#define ATM_BAND_NQ(NS)                                \
-  switch (Nq) {                                        \
-    case 2: return band_solve_mfma_kernel<NS, 2>;      \
-    case 3: return band_solve_mfma_kernel<NS, 3>;      \
-    case 4: return band_solve_mfma_kernel<NS, 4>;      \
-    case 5: return band_solve_mfma_kernel<NS, 5>;      \
-    case 6: return band_solve_mfma_kernel<NS, 6>;      \
-    default: return nullptr;                           \
+BandSolveMfmaFn band_mfma_kernel(int Ns, int Nq, int W) {
+#define ATM_BAND_NQ(NS, WW)                                \
+  switch (Nq) {                                            \
+    case 2: return band_solve_mfma_kernel<NS, 2, WW>;      \
+    case 3: return band_solve_mfma_kernel<NS, 3, WW>;      \
+    case 4: return band_solve_mfma_kernel<NS, 4, WW>;      \
+    case 5: return band_solve_mfma_kernel<NS, 5, WW>;      \
+    case 6: return band_solve_mfma_kernel<NS, 6, WW>;      \
+    default: return nullptr;                               \
   }
-  if (Ns == 1) ATM_BAND_NQ(1)
-  if (Ns == 4) ATM_BAND_NQ(4)
-  if (Ns == 5) ATM_BAND_NQ(5)
+  if (W == 4) {
+    if (Ns == 1) ATM_BAND_NQ(1, 4)
+    if (Ns == 4) ATM_BAND_NQ(4, 4)
+    if (Ns == 5) ATM_BAND_NQ(5, 4)
+  } else if (W == 1) {
+    if (Ns == 1) ATM_BAND_NQ(1, 1)
+    if (Ns == 4) ATM_BAND_NQ(4, 1)
+    if (Ns == 5) ATM_BAND_NQ(5, 1)
+  }
 #undef ATM_BAND_NQ
   return nullptr;
 }
@@ -2196,26 +2217,30 @@ int atm_banded_create(atm_dg* lin, int32_t KV, const int32_t* colelem, int32_t n
     b->line_mat_host[l] = (int32_t)uid[r0];
   }
   b->nu = (int64_t)src.size();
-  // lines grouped by matrix (ascending line order within a group), each group padded to whole waves
+  // lines grouped by matrix (ascending line order within a group), each group padded to whole
+  // runs of 64 lines by repeating the run's first line (the repeated lines compute the same
+  // values and store them to the same place)
   std::vector<int32_t> perm, wmat;
   {
     std::vector<std::vector<int32_t>> by(b->nu);
     for (int64_t l = 0; l < L; ++l) by[b->line_mat_host[l]].push_back((int32_t)l);
     for (int64_t u = 0; u < b->nu; ++u) {
       for (size_t i = 0; i < by[u].size(); i += 64) {
-        for (size_t k = i; k < i + 64; ++k) perm.push_back(k < by[u].size() ? by[u][k] : -1);
+        for (size_t k = i; k < i + 64; ++k) perm.push_back(k < by[u].size() ? by[u][k] : by[u][i]);
         wmat.push_back((int32_t)u);
       }
     }
   }
   b->nwaves = (int64_t)wmat.size();
+  // runs of 16 lines for the block-form solves; a run's padding lanes repeat its first line
+  // (they compute that line's values again and store them to the same place)
   std::vector<int32_t> perm16, wmat16;
   {
     std::vector<std::vector<int32_t>> by(b->nu);
     for (int64_t l = 0; l < L; ++l) by[b->line_mat_host[l]].push_back((int32_t)l);
     for (int64_t u = 0; u < b->nu; ++u)
       for (size_t i = 0; i < by[u].size(); i += 16) {
-        for (size_t k = i; k < i + 16; ++k) perm16.push_back(k < by[u].size() ? by[u][k] : -1);
+        for (size_t k = i; k < i + 16; ++k) perm16.push_back(k < by[u].size() ? by[u][k] : by[u][i]);
         wmat16.push_back((int32_t)u);
       }
   }
@@ -2261,12 +2286,12 @@ int atm_banded_create(atm_dg* lin, int32_t KV, const int32_t* colelem, int32_t n
       b->dev_bytes += blk_bytes;
       hipLaunchKernelGGL(band_blocks_kernel, dim3(b->nu * b->KV), dim3(64), 0, s, g, b->fac, b->blocks);
       if ((e = hipGetLastError()) != hipSuccess) return done(hipfail(ctx, e, "band blocks launch"));
-      if (band_mfma_kernel(lin->Ns, lin->Nq)) {
-        const int BS = b->bw + 1, TI = (BS + 15) / 16, KS = (BS + 3) / 4;
-        const size_t fbytes = (size_t)b->nu * b->KV * 4 * TI * KS * 64 * sizeof(double);
+      if (band_mfma_kernel(lin->Ns, lin->Nq, 1)) {
+        const int BS = b->bw + 1, KT = (BS + 3) / 4;
+        const size_t fbytes = ((size_t)b->nu * b->KV * 4 * KT * KT * 16 + band_mfma_frag_pad(BS)) * sizeof(double);
         if ((e = hipMalloc(&b->frags, fbytes)) != hipSuccess) return done(hipfail(ctx, e, "hipMalloc fragments"));
         b->dev_bytes += fbytes;
-        hipLaunchKernelGGL(band_frags_kernel, dim3(b->nu * b->KV), dim3(256), 0, s, g, b->blocks, b->frags, TI, KS);
+        hipLaunchKernelGGL(band_frags_kernel, dim3(b->nu * b->KV), dim3(256), 0, s, g, b->blocks, b->frags, KT);
         if ((e = hipGetLastError()) != hipSuccess) return done(hipfail(ctx, e, "band fragments launch"));
       }
     }
@@ -2301,13 +2326,24 @@ int atm_banded_destroy(atm_banded* b) {
 
 namespace {
 
+// Waves per workgroup of the matrix-core solve for this factorisation: 4 (64 lines of a
+// matrix sharing each block's fragments in LDS) when that LDS fits, else 1; 0 when the
+// (Ns, Nq) is not instantiated or not even one wave's LDS fits.
+int band_mfma_waves(const atm_banded* b) {
+  const atm_ctx* ctx = b->lin->ctx;
+  for (int W : {4, 1})
+    if (band_mfma_kernel(b->lin->Ns, b->lin->Nq, W) && band_mfma_lds_bytes(b->n, b->KV, b->bw + 1, W) <= ctx->max_lds)
+      return W;
+  return 0;
+}
+
 // Whether solve kernel k can run this factorisation (operand tables built, the
 // (Ns, Nq) instantiated, LDS large enough).
 bool band_kernel_ok(const atm_banded* b, int k) {
   const atm_ctx* ctx = b->lin->ctx;
   switch (k) {
     case ATM_BAND_MFMA:
-      return b->frags && band_mfma_kernel(b->lin->Ns, b->lin->Nq) && band_mfma_lds_bytes(b->n, b->KV) <= ctx->max_lds;
+      return b->frags && band_mfma_waves(b) > 0;
     case ATM_BAND_BLOCK:
       return b->blocks && band_blk_kernel(b->lin->Ns, b->lin->Nq);
     case ATM_BAND_REG:
@@ -2335,11 +2371,17 @@ int band_launch(atm_banded* b, int k, double* x, const double* rhs) {
   const BandGeo g = band_geo(b);
   hipStream_t st = stream_of(ctx);
   switch (k) {
-    case ATM_BAND_MFMA:
-      if (b->nwaves16 > 0)
-        hipLaunchKernelGGL(band_mfma_kernel(b->lin->Ns, b->lin->Nq), dim3(b->nwaves16), dim3(64),
-                           band_mfma_lds_bytes(b->n, b->KV), st, g, b->frags, b->perm16, b->wave_mat16, x, rhs);
+    case ATM_BAND_MFMA: {
+      const int W = band_mfma_waves(b);
+      const size_t lds = band_mfma_lds_bytes(b->n, b->KV, b->bw + 1, W);
+      if (W == 4 && b->nwaves > 0)
+        hipLaunchKernelGGL(band_mfma_kernel(b->lin->Ns, b->lin->Nq, 4), dim3(b->nwaves), dim3(256), lds, st, g, b->frags,
+                           b->perm, b->wave_mat, x, rhs);
+      else if (W == 1 && b->nwaves16 > 0)
+        hipLaunchKernelGGL(band_mfma_kernel(b->lin->Ns, b->lin->Nq, 1), dim3(b->nwaves16), dim3(64), lds, st, g,
+                           b->frags, b->perm16, b->wave_mat16, x, rhs);
       break;
+    }
     case ATM_BAND_BLOCK:
       if (b->nwaves16 > 0)
         hipLaunchKernelGGL(band_blk_kernel(b->lin->Ns, b->lin->Nq), dim3(b->nwaves16), dim3(64), 0, st, g, b->blocks,

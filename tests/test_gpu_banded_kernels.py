@@ -93,3 +93,18 @@ def test_tuner_remembers_the_banded_kernel():
     band2 = A.BandedOperator(c.dg, band.scale)
     hit, t2 = A.autotune_banded(band2, cache=cache)
     assert hit == best and t2 == {} and band2.kernel == best
+
+
+@pytest.mark.parametrize("KV", [16, 18])
+def test_block_form_solves_of_tall_columns(KV):
+    """Columns of 16 elements (the block form's completely unrolled sequence, at its bound) and
+    of 18 (its loop form) against the oracle."""
+    c, band, band_orc = _setup(3, KV)
+    b = c.rng.standard_normal(c.q.shape)
+    ref = c.odg.banded_solve(band_orc, b)
+    for k in ("mfma", "block"):
+        band.kernel = k
+        x = c.dg.fieldarray()
+        band.solve(x, c.dg.fieldarray(b))
+        c.dg.ctx.sync()
+        assert relmax(x.numpy(), ref) <= 1e-11, (k, KV)
