@@ -106,6 +106,18 @@ struct ArkCoef {
   double ea[3], ia[3];
 };
 
+// The right-hand side of an implicit ARK stage, q + dt sum_k<nk (ea_k ex_k + ia_k im_k), formed
+// where it is read (the banded solve's row loads) instead of in a pass of its own; the same
+// operations in the same order as ark_combine_kernel, so the values are the same bits.
+struct ArkRhs {
+  const double* q;
+  const double* e[3];
+  const double* i[3];
+  ArkCoef c;
+  double dt;
+  int unsplit;
+};
+
 // Qhat = q; Qhat += dt*(a_k*exK_k + ã_k*imK_k) for k < nk (src/odesolvers.jl:346-349).
 // unsplit: exK_k holds the full tendency and the explicit part is exK_k - imK_k
 // (src/odesolvers.jl:339-341, 374-376), formed here with the same rounding.
@@ -748,10 +760,10 @@ struct BandMfmaShape {
 
 typedef double atm_v2d __attribute__((ext_vector_type(2)));
 
-template <int NS, int NQ, int W>
+template <int NS, int NQ, int W, int NK>
 __global__ void __launch_bounds__(64 * W)
 band_solve_mfma_kernel(BandGeo g, const double* __restrict__ frags, const int32_t* __restrict__ perm,
-                       const int32_t* __restrict__ run_mat, double* x, const double* b) {
+                       const int32_t* __restrict__ run_mat, double* x, const double* b, ArkRhs ark) {
   using S = BandMfmaShape<NS, NQ, W>;
   constexpr int BS = S::BS, KT = S::KT, NT = S::NT, LDY = S::LDY, FPAD = S::FPAD, CH = S::CH;
   constexpr bool FULL = BS % 4 == 0;  // every lane's rows 4 tk + q lie inside the block
@@ -799,7 +811,21 @@ band_solve_mfma_kernel(BandGeo g, const double* __restrict__ frags, const int32_
 #pragma unroll
     for (int tk = 0; tk < KT; ++tk) {
       const int m = 4 * tk + q;
-      dst[tk] = row_ok(m) ? b[gaddr(ez, m)] : 0.0;
+      if constexpr (NK == 0) {
+        dst[tk] = row_ok(m) ? b[gaddr(ez, m)] : 0.0;
+      } else {  // NK terms of an ARK stage combination (ark_combine_kernel's expression)
+        double r = 0.0;
+        if (row_ok(m)) {
+          const int64_t ad = gaddr(ez, m);
+          r = ark.q[ad];
+#pragma unroll
+          for (int k = 0; k < NK; ++k) {
+            const double im = ark.i[k][ad], ex = ark.unsplit ? ark.e[k][ad] - im : ark.e[k][ad];
+            r = r + ark.dt * (ark.c.ea[k] * ex + ark.c.ia[k] * im);
+          }
+        }
+        dst[tk] = r;
+      }
     }
   };
   auto lds_rows = [&](int ez, double* dst) {
@@ -907,27 +933,40 @@ size_t band_mfma_frag_pad(int BS) {
   return (size_t)2 * KT * KT * 16 + 128;
 }
 
-typedef void (*BandSolveMfmaFn)(BandGeo, const double*, const int32_t*, const int32_t*, double*, const double*);
+typedef void (*BandSolveMfmaFn)(BandGeo, const double*, const int32_t*, const int32_t*, double*, const double*,
+                                ArkRhs);
 
-BandSolveMfmaFn band_mfma_kernel(int Ns, int Nq, int W) {
-#define ATM_BAND_NQ(NS, WW)                                \
-  switch (Nq) {                                            \
-    case 2: return band_solve_mfma_kernel<NS, 2, WW>;      \
-    case 3: return band_solve_mfma_kernel<NS, 3, WW>;      \
-    case 4: return band_solve_mfma_kernel<NS, 4, WW>;      \
-    case 5: return band_solve_mfma_kernel<NS, 5, WW>;      \
-    case 6: return band_solve_mfma_kernel<NS, 6, WW>;      \
-    default: return nullptr;                               \
+// the matrix-core solve for (Ns, Nq), W waves per workgroup, with the right-hand side read
+// (nk = 0) or formed from nk ARK stage terms (nk = 1, 2; the implicit laws' Ns = 4, 5 only)
+BandSolveMfmaFn band_mfma_kernel(int Ns, int Nq, int W, int nk = 0) {
+#define ATM_BAND_NQ(NS, WW, NKK)                                \
+  switch (Nq) {                                                 \
+    case 2: return band_solve_mfma_kernel<NS, 2, WW, NKK>;      \
+    case 3: return band_solve_mfma_kernel<NS, 3, WW, NKK>;      \
+    case 4: return band_solve_mfma_kernel<NS, 4, WW, NKK>;      \
+    case 5: return band_solve_mfma_kernel<NS, 5, WW, NKK>;      \
+    case 6: return band_solve_mfma_kernel<NS, 6, WW, NKK>;      \
+    default: return nullptr;                                    \
   }
-  if (W == 4) {
-    if (Ns == 1) ATM_BAND_NQ(1, 4)
-    if (Ns == 4) ATM_BAND_NQ(4, 4)
-    if (Ns == 5) ATM_BAND_NQ(5, 4)
-  } else if (W == 1) {
-    if (Ns == 1) ATM_BAND_NQ(1, 1)
-    if (Ns == 4) ATM_BAND_NQ(4, 1)
-    if (Ns == 5) ATM_BAND_NQ(5, 1)
+#define ATM_BAND_NS(WW, NKK)                  \
+  if (Ns == 4) ATM_BAND_NQ(4, WW, NKK)        \
+  if (Ns == 5) ATM_BAND_NQ(5, WW, NKK)
+  if (nk == 0) {
+    if (W == 4) {
+      if (Ns == 1) ATM_BAND_NQ(1, 4, 0)
+      ATM_BAND_NS(4, 0)
+    } else if (W == 1) {
+      if (Ns == 1) ATM_BAND_NQ(1, 1, 0)
+      ATM_BAND_NS(1, 0)
+    }
+  } else if (nk == 1) {
+    if (W == 4) ATM_BAND_NS(4, 1)
+    if (W == 1) ATM_BAND_NS(1, 1)
+  } else if (nk == 2) {
+    if (W == 4) ATM_BAND_NS(4, 2)
+    if (W == 1) ATM_BAND_NS(1, 2)
   }
+#undef ATM_BAND_NS
 #undef ATM_BAND_NQ
   return nullptr;
 }
@@ -2366,7 +2405,9 @@ int band_auto_kernel(const atm_banded* b) {
   return ATM_BAND_PLAIN;
 }
 
-int band_launch(atm_banded* b, int k, double* x, const double* rhs) {
+// Solve kernel k on right-hand side rhs, or, with ark (kernel k the matrix-core solve and a
+// fused instance for nk terms present), on the ARK stage combination formed in its row loads.
+int band_launch(atm_banded* b, int k, double* x, const double* rhs, const ArkRhs* ark = nullptr, int nk = 0) {
   atm_ctx* ctx = b->lin->ctx;
   const BandGeo g = band_geo(b);
   hipStream_t st = stream_of(ctx);
@@ -2375,11 +2416,11 @@ int band_launch(atm_banded* b, int k, double* x, const double* rhs) {
       const int W = band_mfma_waves(b);
       const size_t lds = band_mfma_lds_bytes(b->n, b->KV, b->bw + 1, W);
       if (W == 4 && b->nwaves > 0)
-        hipLaunchKernelGGL(band_mfma_kernel(b->lin->Ns, b->lin->Nq, 4), dim3(b->nwaves), dim3(256), lds, st, g, b->frags,
-                           b->perm, b->wave_mat, x, rhs);
+        hipLaunchKernelGGL(band_mfma_kernel(b->lin->Ns, b->lin->Nq, 4, nk), dim3(b->nwaves), dim3(256), lds, st, g,
+                           b->frags, b->perm, b->wave_mat, x, rhs, ark ? *ark : ArkRhs{});
       else if (W == 1 && b->nwaves16 > 0)
-        hipLaunchKernelGGL(band_mfma_kernel(b->lin->Ns, b->lin->Nq, 1), dim3(b->nwaves16), dim3(64), lds, st, g,
-                           b->frags, b->perm16, b->wave_mat16, x, rhs);
+        hipLaunchKernelGGL(band_mfma_kernel(b->lin->Ns, b->lin->Nq, 1, nk), dim3(b->nwaves16), dim3(64), lds, st, g,
+                           b->frags, b->perm16, b->wave_mat16, x, rhs, ark ? *ark : ArkRhs{});
       break;
     }
     case ATM_BAND_BLOCK:
@@ -2536,9 +2577,17 @@ int atm_ark23_run(atm_dg* ex, atm_dg* lin, atm_banded* band, double* q, double* 
           c.ea[k] = exa[i][k];
           c.ia[k] = ima[i][k];
         }
-        hipLaunchKernelGGL(ark_combine_kernel, dim3(blocks), dim3(EW_THREADS), 0, s, Qhat, q, dt, i, c, exK[0],
-                           exK[1], exK[2], imK[0], imK[1], imK[2], n, split_rhs ? 0 : 1);
-        int rc = atm_banded_solve(band, Qs[i], Qhat);
+        const int kern = band->kernel == ATM_BAND_AUTO ? band_auto_kernel(band) : band->kernel;
+        int rc;
+        if (kern == ATM_BAND_MFMA && band_mfma_kernel(lin->Ns, lin->Nq, band_mfma_waves(band), i)) {
+          // the stage combination formed in the solve's row loads (no Qhat pass)
+          const ArkRhs ark{q, {exK[0], exK[1], exK[2]}, {imK[0], imK[1], imK[2]}, c, dt, split_rhs ? 0 : 1};
+          rc = band_launch(band, kern, Qs[i], nullptr, &ark, i);
+        } else {
+          hipLaunchKernelGGL(ark_combine_kernel, dim3(blocks), dim3(EW_THREADS), 0, s, Qhat, q, dt, i, c, exK[0],
+                             exK[1], exK[2], imK[0], imK[1], imK[2], n, split_rhs ? 0 : 1);
+          rc = atm_banded_solve(band, Qs[i], Qhat);
+        }
         if (rc) return rc;
       }
       int rc;
