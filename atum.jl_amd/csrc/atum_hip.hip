@@ -937,7 +937,8 @@ typedef void (*BandSolveMfmaFn)(BandGeo, const double*, const int32_t*, const in
                                 ArkRhs);
 
 // the matrix-core solve for (Ns, Nq), W waves per workgroup, with the right-hand side read
-// (nk = 0) or formed from nk ARK stage terms (nk = 1, 2; the implicit laws' Ns = 4, 5 only)
+// (nk = 0) or formed from one ARK stage term (nk = 1; the implicit laws' Ns = 4, 5 only).  Two
+// terms (five vectors read per row) made the solve slower than a combination pass before it.
 BandSolveMfmaFn band_mfma_kernel(int Ns, int Nq, int W, int nk = 0) {
 #define ATM_BAND_NQ(NS, WW, NKK)                                \
   switch (Nq) {                                                 \
@@ -962,9 +963,6 @@ BandSolveMfmaFn band_mfma_kernel(int Ns, int Nq, int W, int nk = 0) {
   } else if (nk == 1) {
     if (W == 4) ATM_BAND_NS(4, 1)
     if (W == 1) ATM_BAND_NS(1, 1)
-  } else if (nk == 2) {
-    if (W == 4) ATM_BAND_NS(4, 2)
-    if (W == 1) ATM_BAND_NS(1, 2)
   }
 #undef ATM_BAND_NS
 #undef ATM_BAND_NQ
@@ -2580,7 +2578,7 @@ int atm_ark23_run(atm_dg* ex, atm_dg* lin, atm_banded* band, double* q, double* 
         const int kern = band->kernel == ATM_BAND_AUTO ? band_auto_kernel(band) : band->kernel;
         int rc;
         if (kern == ATM_BAND_MFMA && band_mfma_kernel(lin->Ns, lin->Nq, band_mfma_waves(band), i)) {
-          // the stage combination formed in the solve's row loads (no Qhat pass)
+          // the first implicit stage's combination formed in the solve's row loads (no Qhat pass)
           const ArkRhs ark{q, {exK[0], exK[1], exK[2]}, {imK[0], imK[1], imK[2]}, c, dt, split_rhs ? 0 : 1};
           rc = band_launch(band, kern, Qs[i], nullptr, &ark, i);
         } else {
