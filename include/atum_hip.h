@@ -252,7 +252,9 @@ int64_t atm_banded_device_bytes(const atm_banded* b);
 
 /* Solve kernels of atm_banded_solve (all give the solution of the same factors;
  * the block forms agree with substitution to round-off):
- *   MFMA   block form on the matrix cores, 16 lines per wave (Ns in {1,4,5}, Nq <= 6);
+ *   MFMA   block form on the matrix cores in 4x4 FP64 tiles, 16 lines per wave, 64 lines
+ *          of one matrix per 4-wave workgroup sharing each block's operators in LDS (one
+ *          wave per workgroup when that LDS does not fit) (Ns in {1,4,5}, Nq <= 6);
  *   BLOCK  the same block form on the vector units (Ns in {1,4,5}, Nq <= 6);
  *   REG    register-blocked substitution, 64 lines per wave (Ns in {1,4,5}, Nq <= 6);
  *   LDS    substitution with the line matrix staged in LDS (when it fits);
