@@ -108,3 +108,22 @@ def test_block_form_solves_of_tall_columns(KV):
         band.solve(x, c.dg.fieldarray(b))
         c.dg.ctx.sync()
         assert relmax(x.numpy(), ref) <= 1e-11, (k, KV)
+
+
+@pytest.mark.parametrize("N", [3, 4])
+def test_every_kernel_solves_in_place(N):
+    """x may alias the right-hand side (atm_banded_solve): each kernel solving in place gives the
+    oracle's solution."""
+    import atum_hip as A
+    c, band, band_orc = _setup(N)
+    b = c.rng.standard_normal(c.q.shape)
+    ref = c.odg.banded_solve(band_orc, b)
+    for k in KERNELS:
+        try:
+            band.kernel = k
+        except A.AtumError:
+            continue
+        xb = c.dg.fieldarray(b)
+        band.solve(xb, xb)
+        c.dg.ctx.sync()
+        assert relmax(xb.numpy(), ref) <= 1e-11, k
