@@ -771,7 +771,7 @@ band_solve_mfma_kernel(BandGeo g, const double* __restrict__ frags, const int32_
   extern __shared__ double lds[];
   const int KV = g.KV, n = g.n;
   double* yb = lds;                                                 // n x LDY
-  double* fbuf = lds + (size_t)n * LDY;                             // 2 x FPAD
+  double* fbuf = lds + ((size_t)n * LDY + 1) / 2 * 2;               // 2 x FPAD, 16-byte aligned
   int32_t* cl = reinterpret_cast<int32_t*>(fbuf + 2 * FPAD);        // 16 W x KV
   const int tid = threadIdx.x, l = tid & 63, lw = l & 15, q = l >> 4, fe = (l & 3) + 4 * q;
   const int lg = 16 * (tid >> 6) + lw;  // line of the lane in the workgroup
@@ -923,15 +923,14 @@ band_solve_mfma_kernel(BandGeo g, const double* __restrict__ frags, const int32_
 size_t band_mfma_lds_bytes(int n, int KV, int BS, int W) {
   const int KT = (BS + 3) / 4, NT = KT * KT, FSET = 2 * NT * 16;
   const int FPAD = (FSET + 128 * W - 1) / (128 * W) * (128 * W);
-  return (size_t)n * (16 * W + 1) * sizeof(double) + (size_t)2 * FPAD * sizeof(double) +
+  return ((size_t)n * (16 * W + 1) + 1) / 2 * 2 * sizeof(double) + (size_t)2 * FPAD * sizeof(double) +
          (size_t)16 * W * KV * sizeof(int32_t);
 }
 
-// doubles of padding after the fragments: the largest FPAD - FSET over the shapes
-size_t band_mfma_frag_pad(int BS) {
-  const int KT = (BS + 3) / 4;
-  return (size_t)2 * KT * KT * 16 + 128;
-}
+// doubles of padding after the fragments: a workgroup reads a block's fragments in whole
+// rounds of 128 W doubles (FPAD), up to 128 W - 1 past the block's FSET, so the last block's
+// rounds stay inside the allocation with 128 x 4 (the largest W) doubles after it
+size_t band_mfma_frag_pad() { return (size_t)128 * 4; }
 
 typedef void (*BandSolveMfmaFn)(BandGeo, const double*, const int32_t*, const int32_t*, double*, const double*,
                                 ArkRhs);
@@ -2325,7 +2324,7 @@ int atm_banded_create(atm_dg* lin, int32_t KV, const int32_t* colelem, int32_t n
       if ((e = hipGetLastError()) != hipSuccess) return done(hipfail(ctx, e, "band blocks launch"));
       if (band_mfma_kernel(lin->Ns, lin->Nq, 1)) {
         const int BS = b->bw + 1, KT = (BS + 3) / 4;
-        const size_t fbytes = ((size_t)b->nu * b->KV * 4 * KT * KT * 16 + band_mfma_frag_pad(BS)) * sizeof(double);
+        const size_t fbytes = ((size_t)b->nu * b->KV * 4 * KT * KT * 16 + band_mfma_frag_pad()) * sizeof(double);
         if ((e = hipMalloc(&b->frags, fbytes)) != hipSuccess) return done(hipfail(ctx, e, "hipMalloc fragments"));
         b->dev_bytes += fbytes;
         hipLaunchKernelGGL(band_frags_kernel, dim3(b->nu * b->KV), dim3(256), 0, s, g, b->blocks, b->frags, KT);

@@ -127,3 +127,30 @@ def test_every_kernel_solves_in_place(N):
         band.solve(xb, xb)
         c.dg.ctx.sync()
         assert relmax(xb.numpy(), ref) <= 1e-11, k
+
+
+@pytest.mark.parametrize("dim,N,KV", [(2, 1, 5), (2, 2, 3), (3, 1, 4)])
+def test_small_blocks_against_oracle(dim, N, KV):
+    """Blocks of 8 to 12 rows (the matrix-core solve's smallest tiles, whose fragment reads run
+    furthest past a block's data) with every available kernel against the oracle."""
+    import atum_hip as A
+    K = (3,) * (dim - 1) + (KV,)
+    c = Case("linear_euler_gravity", dim, N, K, form="weak", sflux="rusanov", bc="reflect",
+             periodic=(True,) * (dim - 1) + (False,), directions=(dim,), gravity_domain=True,
+             ordering="stacked", seed=N)
+    scale = -5.0 * (1 - 1 / math.sqrt(2))
+    band = A.BandedOperator(c.dg, scale)
+    ref = c.odg.banded_solve(c.odg.banded_factor(c.odg.banded_assemble(), scale),
+                             b := c.rng.standard_normal(c.q.shape))
+    ran = []
+    for k in KERNELS:
+        try:
+            band.kernel = k
+        except A.AtumError:
+            continue
+        x = c.dg.fieldarray()
+        band.solve(x, c.dg.fieldarray(b))
+        c.dg.ctx.sync()
+        assert relmax(x.numpy(), ref) <= 1e-11, k
+        ran.append(k)
+    assert "mfma" in ran
