@@ -111,6 +111,8 @@ _sig("atm_banded_nunique", c_int64, [c_void_p])
 _sig("atm_banded_device_bytes", c_int64, [c_void_p])
 _sig("atm_banded_set_kernel", c_int, [c_void_p, c_int])
 _sig("atm_banded_get_kernel", c_int, [c_void_p])
+_sig("atm_banded_set_waves", c_int, [c_void_p, c_int])
+_sig("atm_banded_get_waves", c_int, [c_void_p])
 _sig("atm_banded_autotune", c_int, [c_void_p, c_void_p, c_void_p, c_int, _i32p, ctypes.POINTER(ctypes.c_float), c_int])
 BAND_KERNELS = {"auto": -1, "mfma": 0, "block": 1, "reg": 2, "lds": 3, "plain": 4}
 _sig("atm_ark23_run", c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_double, c_double,

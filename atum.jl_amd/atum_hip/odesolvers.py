@@ -178,9 +178,21 @@ class BandedOperator:
         self.linrhs.ctx.check(_lib.lib.atm_banded_set_kernel(self.h, _lib.BAND_KERNELS[name]),
                               "atm_banded_set_kernel")
 
+    @property
+    def waves(self) -> int:
+        """Waves per workgroup of the MFMA solve (4: 64 lines share each block's operators in
+        LDS; 1: 16 lines); 0 when the MFMA solve cannot run these factors."""
+        return int(_lib.lib.atm_banded_get_waves(self.h))
+
+    @waves.setter
+    def waves(self, w: int):
+        """4, 1, or 0 for the largest that fits; the solution is the same bits for either."""
+        self.linrhs.ctx.check(_lib.lib.atm_banded_set_waves(self.h, int(w)), "atm_banded_set_waves")
+
     def autotune(self, x: Field, rhs: Field, reps: int = 10):
-        """Time every solve kernel that can run these factors, keep the fastest; returns
-        (name, {name: microseconds per solve}).  x is overwritten."""
+        """Time every solve kernel that can run these factors (the MFMA solve in both of its
+        workgroup shapes), keep the fastest; returns (name, {name: microseconds per solve}).
+        x is overwritten."""
         ks = (ctypes.c_int32 * 8)()
         us = (ctypes.c_float * 8)()
         best = _lib.lib.atm_banded_autotune(self.h, x.ptr, rhs.ptr, int(reps), ks, us, 8)

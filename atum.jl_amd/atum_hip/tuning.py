@@ -1,5 +1,6 @@
 """Autotuner of launch shapes: elements per workgroup of the element kernels and
-the solve kernel of the banded vertical solve.
+the solve kernel of the banded vertical solve (with, for the matrix-core solve, its
+waves per workgroup).
 
 The number of elements per workgroup (epw) of an operator's element kernel
 changes how the device is filled, never the results (each element is computed
@@ -141,15 +142,19 @@ def autotune_banded(band, reps: int = 10, cache: TuneCache | None = None):
     returns (kernel name, {name: us per solve}) (timings empty on a cache hit)."""
     key = banded_key(band)
     if cache is not None and key in cache.table:
+        entry = cache.table[key]
         try:
-            band.kernel = cache.table[key]
+            kernel, waves = (entry["kernel"], entry.get("waves", 0)) if isinstance(entry, dict) else (entry, 0)
+            band.kernel = kernel
+            if kernel == "mfma":
+                band.waves = waves
             return band.kernel, {}
-        except _lib.AtumError:
+        except (_lib.AtumError, KeyError, TypeError):
             pass
     lin = band.linrhs
     rhs = lin.fieldarray()
     rhs.fill_(1.0)
     best, times = band.autotune(lin.fieldarray(), rhs, reps=reps)
     if cache is not None:
-        cache.store(key, best)
+        cache.store(key, {"kernel": best, "waves": band.waves if best == "mfma" else 0})
     return best, times

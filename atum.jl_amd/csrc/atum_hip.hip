@@ -69,6 +69,7 @@ struct atm_banded {
   int32_t* colelem = nullptr;   // ncols*KV
   size_t dev_bytes = 0;         // device memory held (factors, block form, index tables)
   int kernel = ATM_BAND_AUTO;   // solve kernel (ATM_BAND_*)
+  int mfma_w = 0;               // waves per workgroup of the matrix-core solve (0: the largest that fits)
 };
 
 namespace atm {
@@ -2365,11 +2366,15 @@ namespace {
 // Waves per workgroup of the matrix-core solve for this factorisation: 4 (64 lines of a
 // matrix sharing each block's fragments in LDS) when that LDS fits, else 1; 0 when the
 // (Ns, Nq) is not instantiated or not even one wave's LDS fits.
+bool band_mfma_fits(const atm_banded* b, int W) {
+  return band_mfma_kernel(b->lin->Ns, b->lin->Nq, W) &&
+         band_mfma_lds_bytes(b->n, b->KV, b->bw + 1, W) <= b->lin->ctx->max_lds;
+}
+
 int band_mfma_waves(const atm_banded* b) {
-  const atm_ctx* ctx = b->lin->ctx;
+  if (b->mfma_w > 0 && band_mfma_fits(b, b->mfma_w)) return b->mfma_w;  // chosen (set or tuned)
   for (int W : {4, 1})
-    if (band_mfma_kernel(b->lin->Ns, b->lin->Nq, W) && band_mfma_lds_bytes(b->n, b->KV, b->bw + 1, W) <= ctx->max_lds)
-      return W;
+    if (band_mfma_fits(b, W)) return W;
   return 0;
 }
 
@@ -2476,6 +2481,18 @@ int atm_banded_set_kernel(atm_banded* b, int kernel) {
   return ATM_OK;
 }
 
+int atm_banded_set_waves(atm_banded* b, int waves) {
+  if (!b) return ATM_ERR_ARG;
+  if (waves != 0 && waves != 1 && waves != 4)
+    return fail(b->lin->ctx, ATM_ERR_ARG, "atm_banded_set_waves: waves must be 0, 1 or 4");
+  if (waves != 0 && !band_mfma_fits(b, waves))
+    return fail(b->lin->ctx, ATM_ERR_UNSUPPORTED, "atm_banded_set_waves: that workgroup does not fit this factorisation");
+  b->mfma_w = waves;
+  return ATM_OK;
+}
+
+int atm_banded_get_waves(const atm_banded* b) { return b ? band_mfma_waves(b) : ATM_ERR_ARG; }
+
 int atm_banded_autotune(atm_banded* b, double* x, const double* rhs, int reps, int32_t* kernel_out, float* us_out,
                         int max) {
   if (!b || !x || !rhs || reps < 1) return ATM_ERR_ARG;
@@ -2491,17 +2508,36 @@ int atm_banded_autotune(atm_banded* b, double* x, const double* rhs, int reps, i
   }
   int best = -1, n = 0, rc = ATM_OK;
   float best_us = 0.f;
-  for (int k = ATM_BAND_MFMA; k <= ATM_BAND_PLAIN && rc == ATM_OK; ++k) {
-    if (!band_kernel_ok(b, k)) continue;
+  auto timed = [&](int k) -> float {
     rc = band_launch(b, k, x, rhs);  // warm-up
-    if (rc) break;
+    if (rc) return 0.f;
     hipEventRecord(e0, st);
     for (int i = 0; i < reps && rc == ATM_OK; ++i) rc = band_launch(b, k, x, rhs);
     hipEventRecord(e1, st);
     hipEventSynchronize(e1);
     float ms = 0.f;
     hipEventElapsedTime(&ms, e0, e1);
-    const float us = ms * 1e3f / reps;
+    return ms * 1e3f / reps;
+  };
+  for (int k = ATM_BAND_MFMA; k <= ATM_BAND_PLAIN && rc == ATM_OK; ++k) {
+    if (!band_kernel_ok(b, k)) continue;
+    float us = 0.f;
+    if (k == ATM_BAND_MFMA) {  // its second shape dimension: waves per workgroup
+      int best_w = 0;
+      for (int W : {4, 1}) {
+        if (!band_mfma_fits(b, W) || rc) continue;
+        b->mfma_w = W;
+        const float t = timed(k);
+        if (best_w == 0 || t < us) {
+          best_w = W;
+          us = t;
+        }
+      }
+      b->mfma_w = best_w;
+    } else {
+      us = timed(k);
+    }
+    if (rc) break;
     if (kernel_out && n < max) kernel_out[n] = k;
     if (us_out && n < max) us_out[n] = us;
     ++n;

@@ -312,7 +312,7 @@ def imex_workload(args):
                        "dof": dof, "dt": dt, "lines": 32 * 32 * 16, "precision": args.precision,
                        "elements_per_workgroup": {"explicit": ex.epw, "linear": lin.epw},
                        "autotune_us_per_launch": {k: v[1] for k, v in tuned.items()} or None,
-                       "banded_solve_kernel": ark.banded.kernel,
+                       "banded_solve_kernel": ark.banded.kernel, "banded_solve_waves": ark.banded.waves,
                        "banded_autotune_us_per_solve": band_tuned[1] if band_tuned else None}}
 
 

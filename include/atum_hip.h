@@ -274,6 +274,14 @@ int64_t atm_banded_device_bytes(const atm_banded* b);
 #define ATM_BAND_PLAIN 4
 int atm_banded_set_kernel(atm_banded* b, int kernel);
 int atm_banded_get_kernel(const atm_banded* b);
+/* Waves per workgroup of the MFMA solve: 4 (64 lines of one matrix share each block's
+ * operators in LDS) or 1 (16 lines); 0 = the largest whose LDS fits.  atm_banded_set_waves
+ * fails with ATM_ERR_UNSUPPORTED for a shape that does not fit this factorisation;
+ * atm_banded_get_waves returns the shape a solve would use (0: no MFMA solve here).  The
+ * results are the same bits for either shape.  atm_banded_autotune times both shapes of the
+ * MFMA solve, keeps the faster one and reports its time.  No counterpart in the reference. */
+int atm_banded_set_waves(atm_banded* b, int waves);
+int atm_banded_get_waves(const atm_banded* b);
 int atm_banded_autotune(atm_banded* b, double* x, const double* rhs, int reps, int32_t* kernel_out, float* us_out,
                         int max);
 /* x = A^{-1} b for every line (ldiv!, src/odesolvers.jl:355); x may alias b. */

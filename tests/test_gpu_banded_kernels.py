@@ -154,3 +154,37 @@ def test_small_blocks_against_oracle(dim, N, KV):
         assert relmax(x.numpy(), ref) <= 1e-11, k
         ran.append(k)
     assert "mfma" in ran
+
+
+def test_mfma_workgroup_shapes_give_the_same_bits():
+    """The MFMA solve with 4 waves per workgroup (64 lines sharing each block's operators in
+    LDS) and with 1 (16 lines) gives the same bits, matching the oracle; a shape whose LDS does
+    not fit is refused; the tuner picks one of the two and the cache restores it."""
+    import atum_hip as A
+    c, band, band_orc = _setup(3)
+    b = c.rng.standard_normal(c.q.shape)
+    ref = c.odg.banded_solve(band_orc, b)
+    band.kernel = "mfma"
+    out = {}
+    for w in (4, 1):
+        band.waves = w
+        assert band.waves == w
+        x = c.dg.fieldarray()
+        band.solve(x, c.dg.fieldarray(b))
+        c.dg.ctx.sync()
+        out[w] = x.numpy()
+    assert np.array_equal(out[4], out[1])
+    assert relmax(out[4], ref) <= 1e-11
+    band.waves = 0
+    assert band.waves == 4  # the largest that fits
+    cache = A.TuneCache()
+    best, times = A.autotune_banded(band, reps=2, cache=cache)
+    assert best == "mfma" and band.waves in (1, 4)
+    band2 = A.BandedOperator(c.dg, band.scale)
+    hit, t2 = A.autotune_banded(band2, cache=cache)
+    assert hit == "mfma" and t2 == {} and band2.waves == band.waves
+    # a column of 18 elements (n = 360) leaves no room for four waves' forward solutions in LDS
+    tall, tband, _ = _setup(3, 18)
+    with pytest.raises(A.AtumError):
+        tband.waves = 4
+    assert tband.waves == 1

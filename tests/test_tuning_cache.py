@@ -23,3 +23,11 @@ def test_cache_round_trip(tmp_path):
     assert c.get(_fake(("euler", 2, 4, 1, 2, 1, 3, "fast"), 4096)) is None  # other size class
     assert c.get(_fake(("euler", 2, 5, 1, 2, 1, 3, "fast"), 64)) is None    # other N
     assert TuneCache(path).get(a) == 5
+
+
+def test_banded_entry_round_trip(tmp_path):
+    """A banded solve choice is stored with the MFMA solve's waves per workgroup."""
+    path = str(tmp_path / "b.json")
+    c = TuneCache(path)
+    c.store("band|x", {"kernel": "mfma", "waves": 4})
+    assert TuneCache(path).table["band|x"] == {"kernel": "mfma", "waves": 4}
