@@ -752,6 +752,8 @@ __global__ void band_frags_kernel(BandGeo g, const double* __restrict__ blocks, 
 // buffer stays in place across iterations, so the compiler never waits for all
 // outstanding loads.  Padding lanes of a run repeat the run's first line: they
 // compute the same values and store them to the same place, so every lane stores.
+// With NK > 0 the right-hand side is not read from b but formed in the row loads from
+// NK terms of an ARK stage combination (ArkRhs).
 template <int NS, int NQ, int W>
 struct BandMfmaShape {
   static constexpr int BS = NS * NQ, KT = (BS + 3) / 4, NT = KT * KT, LDY = 16 * W + 1;
