@@ -61,6 +61,12 @@ constexpr int fast_xbuf() {
   return a > b ? a : b;
 }
 
+// The lean register and LDS form of the kernel (fd_fast_body) for 3-D elements of N >= 4.
+template <class L, int NQ>
+constexpr bool fast_lean() {
+  return L::D == 3 && NQ >= 5;
+}
+
 // Per-node primitives: [rho, u (D), b, u.u, phi, 1/b]
 template <int D, bool GRAV>
 struct FastEC {
@@ -161,7 +167,16 @@ ATM_DEV void fd_fast_body(const DevDG& p, double* lds, int64_t blk, const double
   constexpr int Np = D == 1 ? NQ : (D == 2 ? NQ * NQ : NQ * NQ * NQ);
   constexpr int M = Nq / 2;  // pair slots per node and direction
   double* sD = lds;
-  constexpr int stride = (NP + D) * Np + fast_xbuf<L, NQ>();
+  // The lean form (3-D elements of N >= 4): the LDS primitives without u.u (recomputed from
+  // u where a partner's are read), each later direction's metric row loaded at that
+  // direction, and the stage's dq, MJI and the state for the update loaded at the end, so
+  // that no register holds them through the volume term.  It brings the 3-D N = 4
+  // Euler-gravity kernel from 150 to 128 VGPRs and 20 KB of LDS per element: 4 waves per
+  // SIMD instead of 3.  For smaller elements the workgroup's life is too short to hide the
+  // loads at the end (measured slower), and they keep every load at the start.
+  constexpr bool LEAN = fast_lean<L, NQ>();
+  constexpr int NPL = LEAN ? NP - 1 : NP;  // LDS planes of primitives
+  constexpr int stride = (NPL + D) * Np + fast_xbuf<L, NQ>();
   const int tid = threadIdx.x;
   const int el = tid / Np;
   const int ijk = tid - el * Np;
@@ -169,7 +184,7 @@ ATM_DEV void fd_fast_body(const DevDG& p, double* lds, int64_t blk, const double
   const bool active = el < epw && le < p.K;
   const int64_t e = (p.sched && active) ? (int64_t)p.sched[le] : le;
   double* sP = lds + Nq * Nq + el * stride;  // [k][node]
-  double* slg = sP + NP * Np;                // [d][node]
+  double* slg = sP + NPL * Np;               // [d][node]
   double* sx = slg + D * Np;                 // [m][s][node]
   for (int t = tid; t < Nq * Nq; t += blockDim.x) sD[t] = p.D[t];
   double q1[Ns], a1[NA], P1[NP];
@@ -184,23 +199,26 @@ ATM_DEV void fd_fast_body(const DevDG& p, double* lds, int64_t blk, const double
   for (int s = 0; s < Ns; ++s) q1[s] = q[s * p.Ntot + nl];
   load_aux<L>(p, nl, a1);
   const double MJn = p.MJ[nl];
-  const double MJIn = p.MJI[nl];
-  double dq0[Ns];
+  double MJIn = 0.0, dq0[Ns];
 #pragma unroll
   for (int s = 0; s < Ns; ++s) dq0[s] = 0.0;
-  if (!assign) {  // uniform over the launch
+  double gall[LEAN ? 1 : D][D];  // metric rows: the first direction's only in the lean form
+  if constexpr (!LEAN) {
+    MJIn = p.MJI[nl];
+    if (!assign) {  // uniform over the launch
 #pragma unroll
-    for (int s = 0; s < Ns; ++s) dq0[s] = dq[s * p.Ntot + nl];
+      for (int s = 0; s < Ns; ++s) dq0[s] = dq[s * p.Ntot + nl];
+    }
   }
-  double gall[D][D];
 #pragma unroll
-  for (int r = 0; r < D; ++r)
+  for (int r = 0; r < (LEAN ? 1 : D); ++r)
 #pragma unroll
     for (int d = 0; d < D; ++d) gall[r][d] = p.g[(int64_t)(r + D * d) * p.Ntot + nl];
   FE::prim(p.c, q1, a1, P1);
   if (active) {
 #pragma unroll
-    for (int k = 0; k < NP; ++k) sP[k * Np + ijk] = P1[k];
+    for (int k = 0; k < NP; ++k)
+      if (!LEAN || k != D + 2) sP[(!LEAN || k < D + 2 ? k : k - 1) * Np + ijk] = P1[k];
   }
   int idx[3];
   node_index<D>(ijk, Nq, idx);
@@ -211,6 +229,11 @@ ATM_DEV void fd_fast_body(const DevDG& p, double* lds, int64_t blk, const double
   double src[Ns];
 #pragma unroll
   for (int s = 0; s < Ns; ++s) src[s] = 0.0;
+  if (LEAN && active) {  // every direction's sources first, in direction order
+#pragma unroll
+    for (int r = 0; r < D; ++r)
+      if ((p.dirmask >> r) & 1) L::sources(p.c, src, q1, a1, 1 << r, false);
+  }
 #pragma unroll
   for (int r = 0; r < D; ++r) {
     if (!((p.dirmask >> r) & 1)) continue;
@@ -218,16 +241,19 @@ ATM_DEV void fd_fast_body(const DevDG& p, double* lds, int64_t blk, const double
     // direction's post-pair barrier, so all reads of the old slg are done; the
     // slots are rewritten only after the next barrier, which every thread
     // reaches after gathering its previous slots.
+    double gr[D];
+#pragma unroll
+    for (int d = 0; d < D; ++d) gr[d] = (!LEAN || r == 0) ? gall[LEAN ? 0 : r][d] : p.g[(int64_t)(r + D * d) * p.Ntot + nl];
     if (active) {
 #pragma unroll
-      for (int d = 0; d < D; ++d) slg[d * Np + ijk] = MJn * gall[r][d];
+      for (int d = 0; d < D; ++d) slg[d * Np + ijk] = MJn * gr[d];
     }
     __syncthreads();
     const int a = idx[r];
     const int lstride = (r == 0) ? 1 : ((r == 1) ? Nq : Nq * Nq);
     const int lbase = ijk - a * lstride;
     if (active) {
-      L::sources(p.c, src, q1, a1, 1 << r, false);
+      if constexpr (!LEAN) L::sources(p.c, src, q1, a1, 1 << r, false);
       double lga[D];
 #pragma unroll
       for (int d = 0; d < D; ++d) lga[d] = slg[d * Np + ijk];
@@ -238,7 +264,14 @@ ATM_DEV void fd_fast_body(const DevDG& p, double* lds, int64_t blk, const double
         const int ib = lbase + b * lstride;
         double P2[NP];
 #pragma unroll
-        for (int k = 0; k < NP; ++k) P2[k] = sP[k * Np + ib];
+        for (int k = 0; k < NP; ++k)
+          if (!LEAN || k != D + 2) P2[k] = sP[(!LEAN || k < D + 2 ? k : k - 1) * Np + ib];
+        if constexpr (LEAN) {  // the partner's u.u, the same operations as in prim()
+          double uu = P2[1] * P2[1];
+#pragma unroll
+          for (int d = 1; d < D; ++d) uu += P2[1 + d] * P2[1 + d];
+          P2[D + 2] = uu;
+        }
         typename FE::Pair w;
         FE::pair(p.c, inv_gm1, P1, P2, w);
         const double Dba = sD[b + Nq * a], Dab = sD[a + Nq * b];
@@ -275,6 +308,13 @@ ATM_DEV void fd_fast_body(const DevDG& p, double* lds, int64_t blk, const double
   if (active) surface_faces_lds<L, SF, true>(p, Nq, Np, e, ijk, q, sx);
   __syncthreads();
   if (!active) return;
+  if constexpr (LEAN) {
+#pragma unroll
+    for (int s = 0; s < Ns; ++s) dq0[s] = assign ? 0.0 : dq[s * p.Ntot + node];
+    MJIn = p.MJI[node];
+#pragma unroll
+    for (int s = 0; s < Ns; ++s) q1[s] = q[s * p.Ntot + node];
+  }
   double acc[Ns];
 #pragma unroll
   for (int s = 0; s < Ns; ++s) {
@@ -368,7 +408,7 @@ template <class L, int NQ>
 constexpr size_t fast_lds_elem_bytes() {
   constexpr int D = L::D, NP = FastEC<L::D, true>::NP;
   constexpr int Np = D == 1 ? NQ : (D == 2 ? NQ * NQ : NQ * NQ * NQ);
-  return sizeof(double) * ((size_t)(NP + D) * Np + fast_xbuf<L, NQ>());
+  return sizeof(double) * ((size_t)(NP - (fast_lean<L, NQ>() ? 1 : 0) + D) * Np + fast_xbuf<L, NQ>());
 }
 
 template <class L, int NQ, int SF>
