@@ -125,75 +125,6 @@ ATM_DEV void surface_node(const DevDG& p, int Nq, int64_t e, const int idx[3], c
   }
 }
 
-// Face-point-parallel surface term: the threads of an element deal out its face
-// points (faces of the directions in the mask), each writes faceMJ*nf of its face
-// point to buf[s * nfp + f], f = face*Nfpf + fp; surface_gather then subtracts a
-// node's slots times its MJI in direction order, as the node-centric form does
-// (the same products, (faceMJ*nf)*MJI, in the same order).
-template <class L, int SF, bool FAST>
-ATM_DEV void surface_faces_lds(const DevDG& p, int Nq, int Np, int64_t e, int t, const double* q, double* buf) {
-  constexpr int D = L::D, Ns = L::Ns, NA = L::NA;
-  const int Nfpf = p.Nfpf, nfp = 2 * D * Nfpf;
-  for (int f = t; f < nfp; f += Np) {
-    const int face = f / Nfpf, fp = f - face * Nfpf, r = face >> 1, side = face & 1;
-    if (!((p.dirmask >> r) & 1)) continue;
-    int id[3] = {0, 0, 0}, rem = fp;
-#pragma unroll
-    for (int d = 0; d < D; ++d) {
-      if (d == r) continue;
-      id[d] = rem % Nq;
-      rem /= Nq;
-    }
-    id[r] = side ? Nq - 1 : 0;
-    const int64_t node = e * Np + id[0] + Nq * (id[1] + Nq * id[2]);
-    const int64_t jj = e * p.Nfp + f;
-    // every load first and unconditionally (a boundary face point reads its own node as
-    // the neighbour and replaces it by the boundary state below), so that none waits for
-    // another on a conditional path
-    const int tag = p.bctag[e * (2 * D) + face];
-    const int64_t idx_p = p.faceix_p[jj];
-    const int64_t idp = tag == 0 ? idx_p : node;
-    double qm[Ns], am[NA], n[D], qp[Ns], ap[NA], nf[Ns];
-#pragma unroll
-    for (int s = 0; s < Ns; ++s) qm[s] = q[s * p.Ntot + node];
-    load_aux<L>(p, node, am);
-#pragma unroll
-    for (int d = 0; d < D; ++d) n[d] = p.normal[d * p.NF + jj];
-    const double fMJ = p.faceMJ[jj];
-#pragma unroll
-    for (int s = 0; s < Ns; ++s) qp[s] = q[s * p.Ntot + idp];
-    load_aux<L>(p, idp, ap);
-    if (tag != 0) {
-      boundarystate<L>(p.c, n, qm, tag, qp);
-#pragma unroll
-      for (int s = 0; s < NA; ++s) ap[s] = am[s];
-    }
-    if constexpr (FAST && has_fast_sflux<L, SF>()) fast_sflux<L, SF>(p.c, n, qm, am, qp, ap, nf);
-    else dispatch_sflux<L, SF>(p, n, qm, am, qp, ap, nf);
-#pragma unroll
-    for (int s = 0; s < Ns; ++s) buf[s * nfp + f] = fMJ * nf[s];
-  }
-}
-
-template <class L>
-ATM_DEV void surface_gather(const DevDG& p, int Nq, const int idx[3], const double* buf, double* acc, double MJIn) {
-  constexpr int D = L::D, Ns = L::Ns;
-  const int Nfpf = p.Nfpf, nfp = 2 * D * Nfpf;
-#pragma unroll
-  for (int r = 0; r < D; ++r) {
-    if (!((p.dirmask >> r) & 1)) continue;
-    const int ir = idx[r];
-    const int side = (ir == 0) ? 0 : ((ir == Nq - 1) ? 1 : -1);
-    if (side < 0) continue;
-    int fp = 0;
-    if (D == 2) fp = (r == 0) ? idx[1] : idx[0];
-    if (D == 3) fp = (r == 0) ? idx[1] + Nq * idx[2] : ((r == 1) ? idx[0] + Nq * idx[2] : idx[0] + Nq * idx[1]);
-    const int f = (2 * r + side) * Nfpf + fp;
-#pragma unroll
-    for (int s = 0; s < Ns; ++s) acc[s] -= buf[s * nfp + f] * MJIn;
-  }
-}
-
 template <int D>
 ATM_DEV void node_index(int ijk, int Nq, int idx[3]) {
   idx[0] = ijk % Nq;

@@ -156,6 +156,93 @@ struct FastEC {
   }
 };
 
+// The surface term of the FAST kernels: an element's face points dealt out over its threads
+// (point f = t + i Np in round i, faces of the directions in the mask), each writing faceMJ*nf
+// to buf[s * NFP + f], f = face*NFPF + fp; gather() then subtracts a node's slots times its MJI
+// in direction order, as the node-centric form of the reference-order kernels does (the same
+// products, (faceMJ*nf)*MJI, in the same order).  Everything that indexes a face point is
+// compile-time.  The element's own state and geopotential are read from LDS, where the
+// threads staged them by node; each round's boundary tag and neighbour index are loaded by
+// load() before the barrier that precedes run(), so the neighbour's state is the only global
+// load on the path to each flux.
+template <class L, int SF, int NQ>
+struct FastFaces {
+  static constexpr int D = L::D, Ns = L::Ns, NA = L::NA;
+  static constexpr int Np = D == 1 ? NQ : (D == 2 ? NQ * NQ : NQ * NQ * NQ);
+  static constexpr int NFPF = Np / NQ, NFP = 2 * D * NFPF;
+  static constexpr int R = (NFP + Np - 1) / Np;  // rounds
+  int tag[R];
+  int64_t nb[R];
+  // a round past the last face point reads the last face point's entries and ignores them
+  ATM_DEV void load(const DevDG& p, int64_t e, int t) {
+#pragma unroll
+    for (int i = 0; i < R; ++i) {
+      const int f = min(t + i * Np, NFP - 1);
+      tag[i] = p.bctag[e * (2 * D) + f / NFPF];
+      nb[i] = p.faceix_p[e * p.Nfp + f];
+    }
+  }
+  // sQ: [s][node] the element's state (s < Ns) and geopotential (s = Ns)
+  ATM_DEV void run(const DevDG& p, int64_t e, int t, const double* q, double* buf, const double* sQ) const {
+#pragma unroll
+    for (int i = 0; i < R; ++i) {
+      const int f = t + i * Np;
+      if (f >= NFP) break;
+      const int face = f / NFPF, fp = f - face * NFPF, r = face >> 1, side = face & 1;
+      if (!((p.dirmask >> r) & 1)) continue;
+      int id[3] = {0, 0, 0}, rem = fp;
+#pragma unroll
+      for (int d = 0; d < D; ++d) {
+        if (d == r) continue;
+        id[d] = rem % NQ;
+        rem /= NQ;
+      }
+      id[r] = side ? NQ - 1 : 0;
+      const int ln = id[0] + NQ * (id[1] + NQ * id[2]);
+      const int64_t node = e * Np + ln;
+      const int64_t jj = e * p.Nfp + f;
+      // a boundary face point reads its own node as the neighbour (every load unconditional)
+      // and replaces it by the boundary state below
+      const int64_t idp = tag[i] == 0 ? nb[i] : node;
+      double qm[Ns], am[NA], n[D], qp[Ns], ap[NA], nf[Ns];
+#pragma unroll
+      for (int s = 0; s < Ns; ++s) qm[s] = sQ[s * Np + ln];
+      load_aux<L>(p, node, am);
+      am[0] = sQ[Ns * Np + ln];
+#pragma unroll
+      for (int d = 0; d < D; ++d) n[d] = p.normal[d * p.NF + jj];
+      const double fMJ = p.faceMJ[jj];
+#pragma unroll
+      for (int s = 0; s < Ns; ++s) qp[s] = q[s * p.Ntot + idp];
+      load_aux<L>(p, idp, ap);
+      if (tag[i] != 0) {
+        boundarystate<L>(p.c, n, qm, tag[i], qp);
+#pragma unroll
+        for (int s = 0; s < NA; ++s) ap[s] = am[s];
+      }
+      if constexpr (has_fast_sflux<L, SF>()) fast_sflux<L, SF>(p.c, n, qm, am, qp, ap, nf);
+      else dispatch_sflux<L, SF>(p, n, qm, am, qp, ap, nf);
+#pragma unroll
+      for (int s = 0; s < Ns; ++s) buf[s * NFP + f] = fMJ * nf[s];
+    }
+  }
+  ATM_DEV static void gather(const DevDG& p, const int idx[3], const double* buf, double* acc, double MJIn) {
+#pragma unroll
+    for (int r = 0; r < D; ++r) {
+      if (!((p.dirmask >> r) & 1)) continue;
+      const int ir = idx[r];
+      const int side = (ir == 0) ? 0 : ((ir == NQ - 1) ? 1 : -1);
+      if (side < 0) continue;
+      int fp = 0;
+      if (D == 2) fp = (r == 0) ? idx[1] : idx[0];
+      if (D == 3) fp = (r == 0) ? idx[1] + NQ * idx[2] : ((r == 1) ? idx[0] + NQ * idx[2] : idx[0] + NQ * idx[1]);
+      const int f = (2 * r + side) * NFPF + fp;
+#pragma unroll
+      for (int s = 0; s < Ns; ++s) acc[s] -= buf[s * NFP + f] * MJIn;
+    }
+  }
+};
+
 template <class L, int NQ, int SF>
 ATM_DEV void fd_fast_body(const DevDG& p, double* lds, int64_t blk, const double* __restrict__ q, double* __restrict__ dq, double alpha,
                int assign, double* __restrict__ qout, double bdt, int epw) {
@@ -177,6 +264,7 @@ ATM_DEV void fd_fast_body(const DevDG& p, double* lds, int64_t blk, const double
   constexpr bool LEAN = fast_lean<L, NQ>();
   constexpr int NPL = LEAN ? NP - 1 : NP;  // LDS planes of primitives
   constexpr int stride = (NPL + D) * Np + fast_xbuf<L, NQ>();
+  static_assert(NPL + D >= Ns + 1, "the surface term stages the state and geopotential in the primitives' planes");
   const int tid = threadIdx.x;
   const int el = tid / Np;
   const int ijk = tid - el * Np;
@@ -303,9 +391,21 @@ ATM_DEV void fd_fast_body(const DevDG& p, double* lds, int64_t blk, const double
       }
     }
   }
-  // surface term: face points dealt out over the element's threads, results in LDS
+  // Surface term.  No thread reads a partner's primitives after the last direction's barrier,
+  // so the element's state and geopotential are staged by node in the primitives' LDS planes
+  // (the lean form reloads the state with one coalesced load); the face points read their own
+  // side there instead of with strided global loads, and the update reads the state back.
+  using FF = FastFaces<L, SF, NQ>;
+  FF ff;
+  ff.load(p, active ? e : 0, ijk);
+  double* sQ = sP;
+  if (active) {
+#pragma unroll
+    for (int s = 0; s < Ns; ++s) sQ[s * Np + ijk] = LEAN ? q[s * p.Ntot + node] : q1[s];
+    sQ[Ns * Np + ijk] = P1[D + 3];
+  }
   __syncthreads();  // every thread has consumed its exchange slots
-  if (active) surface_faces_lds<L, SF, true>(p, Nq, Np, e, ijk, q, sx);
+  if (active) ff.run(p, e, ijk, q, sx, sQ);
   __syncthreads();
   if (!active) return;
   if constexpr (LEAN) {
@@ -313,7 +413,7 @@ ATM_DEV void fd_fast_body(const DevDG& p, double* lds, int64_t blk, const double
     for (int s = 0; s < Ns; ++s) dq0[s] = assign ? 0.0 : dq[s * p.Ntot + node];
     MJIn = p.MJI[node];
 #pragma unroll
-    for (int s = 0; s < Ns; ++s) q1[s] = q[s * p.Ntot + node];
+    for (int s = 0; s < Ns; ++s) q1[s] = sQ[s * Np + ijk];
   }
   double acc[Ns];
 #pragma unroll
@@ -321,7 +421,7 @@ ATM_DEV void fd_fast_body(const DevDG& p, double* lds, int64_t blk, const double
     const double v = src[s] + MJIn * vol[s];
     acc[s] = assign ? v : alpha * dq0[s] + v;
   }
-  surface_gather<L>(p, Nq, idx, sx, acc, MJIn);
+  FF::gather(p, idx, sx, acc, MJIn);
   epilogue<L>(p, node, q1, acc, dq, qout, bdt);
 }
 
