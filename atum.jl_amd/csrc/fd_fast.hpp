@@ -51,13 +51,16 @@ __device__ __forceinline__ double lm_series(double u) {
 
 constexpr double LM_U0 = 0.01;
 
-// doubles per element of the LDS exchange buffer: pair shares (M*Ns per node)
-// and, afterwards, the face-point surface results (Ns per face point)
+// doubles per element of the LDS exchange buffer: pair shares (M = NQ/2 slots of Ns per
+// node; for even NQ the last slot, at pair distance NQ/2, receives shares only in the upper
+// half of each line and holds Ns per node of that half) and, afterwards, the face-point
+// surface results (Ns per face point)
 template <class L, int NQ>
 constexpr int fast_xbuf() {
   constexpr int D = L::D, Ns = L::Ns;
   constexpr int Np = D == 1 ? NQ : (D == 2 ? NQ * NQ : NQ * NQ * NQ);
-  constexpr int a = (NQ / 2) * Ns * Np, b = 2 * D * (Np / NQ) * Ns;
+  constexpr int a = NQ % 2 == 0 ? (NQ / 2 - 1) * Ns * Np + Ns * (Np / 2) : (NQ / 2) * Ns * Np;
+  constexpr int b = 2 * D * (Np / NQ) * Ns;
   return a > b ? a : b;
 }
 
@@ -182,7 +185,7 @@ struct FastFaces {
       nb[i] = p.faceix_p[e * p.Nfp + f];
     }
   }
-  // sQ: [s][node] the element's state (s < Ns) and geopotential (s = Ns)
+  // sQ: [s][node] the element's state (s < Ns) and, for Euler-gravity, its geopotential (s = Ns)
   ATM_DEV void run(const DevDG& p, int64_t e, int t, const double* q, double* buf, const double* sQ) const {
 #pragma unroll
     for (int i = 0; i < R; ++i) {
@@ -208,7 +211,7 @@ struct FastFaces {
 #pragma unroll
       for (int s = 0; s < Ns; ++s) qm[s] = sQ[s * Np + ln];
       load_aux<L>(p, node, am);
-      am[0] = sQ[Ns * Np + ln];
+      if constexpr (L::LAW == ATM_LAW_EULER_GRAVITY) am[0] = sQ[Ns * Np + ln];
 #pragma unroll
       for (int d = 0; d < D; ++d) n[d] = p.normal[d * p.NF + jj];
       const double fMJ = p.faceMJ[jj];
@@ -310,6 +313,18 @@ ATM_DEV void fd_fast_body(const DevDG& p, double* lds, int64_t blk, const double
   }
   int idx[3];
   node_index<D>(ijk, Nq, idx);
+  // the last exchange slot of even Nq (pair distance M): a node of the upper half of its line
+  // in direction r, at line position c >= M, by its position in that half
+  constexpr bool HALF = Nq % 2 == 0;
+  auto half_pos = [&](int r, int c) {
+    int pos = 0, st = 1;
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+      pos += (d == r ? c - M : idx[d]) * st;
+      st *= d == r ? M : Nq;
+    }
+    return pos;
+  };
   const double inv_gm1 = 1 / (p.c.gamma - 1);
   double vol[Ns];
 #pragma unroll
@@ -373,11 +388,13 @@ ATM_DEV void fd_fast_body(const DevDG& p, double* lds, int64_t blk, const double
 #pragma unroll
         for (int d = 0; d < D; ++d) vol[1 + d] -= G[d] * w.fla;
         // the partner's share: -G.F, plus its own gravity fluctuation term
-        double* slot = sx + (m - 1) * Ns * Np + ib;
+        const bool hs = HALF && m == M;
+        double* slot = sx + (m - 1) * Ns * Np + (hs ? half_pos(r, b) : ib);
+        const int xs = hs ? Np / 2 : Np;
         slot[0] = -gf[0];
 #pragma unroll
-        for (int d = 0; d < D; ++d) slot[(1 + d) * Np] = G[d] * w.flb - gf[1 + d];
-        slot[(D + 1) * Np] = -gf[D + 1];
+        for (int d = 0; d < D; ++d) slot[(1 + d) * xs] = G[d] * w.flb - gf[1 + d];
+        slot[(D + 1) * xs] = -gf[D + 1];
       }
     }
     __syncthreads();
@@ -385,9 +402,11 @@ ATM_DEV void fd_fast_body(const DevDG& p, double* lds, int64_t blk, const double
 #pragma unroll
       for (int m = 1; m <= M; ++m) {
         if ((Nq % 2 == 0) && m == M && a < M) continue;
-        const double* slot = sx + (m - 1) * Ns * Np + ijk;
+        const bool hs = HALF && m == M;
+        const double* slot = sx + (m - 1) * Ns * Np + (hs ? half_pos(r, a) : ijk);
+        const int xs = hs ? Np / 2 : Np;
 #pragma unroll
-        for (int s = 0; s < Ns; ++s) vol[s] += slot[s * Np];
+        for (int s = 0; s < Ns; ++s) vol[s] += slot[s * xs];
       }
     }
   }
@@ -402,7 +421,7 @@ ATM_DEV void fd_fast_body(const DevDG& p, double* lds, int64_t blk, const double
   if (active) {
 #pragma unroll
     for (int s = 0; s < Ns; ++s) sQ[s * Np + ijk] = LEAN ? q[s * p.Ntot + node] : q1[s];
-    sQ[Ns * Np + ijk] = P1[D + 3];
+    if constexpr (GRAV) sQ[Ns * Np + ijk] = P1[D + 3];  // = aux[0]
   }
   __syncthreads();  // every thread has consumed its exchange slots
   if (active) ff.run(p, e, ijk, q, sx, sQ);
