@@ -51,7 +51,7 @@ def test_fast_euler(dim, N, sflux):
 
 
 @pytest.mark.parametrize("dim", [2, 3])
-@pytest.mark.parametrize("N", [2, 3, 4, 5])
+@pytest.mark.parametrize("N", [1, 2, 3, 4, 5, 6, 7])
 @pytest.mark.parametrize("sflux", ["roe", "rusanov"])
 def test_fast_euler_gravity(dim, N, sflux):
     c = _fast_case("euler_gravity", dim, N, sflux, bc="reflect", periodic=(False,) * dim,
