@@ -160,6 +160,7 @@ ATM_DEV void fd_body(const DevDG& p, double* lds, int64_t blk, const double* __r
           int assign, double* __restrict__ qout, double bdt, int epw) {
   using PF = PairFlux<L, VF>;
   constexpr int D = L::D, Ns = L::Ns, NA = L::NA, NP = PF::NP;
+  constexpr int FD_PARTNER_UNROLL = D == 3 ? 1 : 8;
   const int Nq = NQ > 0 ? NQ : p.Nq;
   const int Np = NQ > 0 ? (D == 1 ? NQ : (D == 2 ? NQ * NQ : NQ * NQ * NQ)) : p.Np;
   double* sD = lds;
@@ -273,6 +274,9 @@ ATM_DEV void fd_body(const DevDG& p, double* lds, int64_t blk, const double* __r
     double lgi[D];
 #pragma unroll
     for (int d = 0; d < D; ++d) lgi[d] = slg[d * Np + ijk];
+    // the partner loop of the 3-D kernels is not unrolled: unrolled, the small-Nq kernels hold
+    // several partners' primitives at once and drop to 2 waves per SIMD (Nq = 4: 210 VGPRs)
+#pragma unroll FD_PARTNER_UNROLL
     for (int n = 0; n < Nq; ++n) {
       const int ild = lbase + n * lstride;
       double P2[NP];
