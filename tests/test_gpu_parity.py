@@ -70,6 +70,15 @@ def test_euler_gravity_tendency(dim, form, sflux, N):
                          periodic=(False,) * dim, gravity_domain=True, seed=7))
 
 
+@pytest.mark.parametrize("N", [1, 2, 3, 4, 5, 6, 7])
+@pytest.mark.parametrize("sflux", ["roe", "rusanov"])
+def test_euler_gravity_ec_every_degree(sflux, N):
+    """Reference precision, 3-D EC flux differencing with walls: the kernels specialised at every
+    degree for the Roe and Rusanov surface fluxes (the bubble and IMEX configurations)."""
+    _check_tendency(Case("euler_gravity", 3, N, 2, form="ec", sflux=sflux, bc="reflect",
+                         periodic=(False,) * 3, gravity_domain=True, seed=11 + N))
+
+
 def test_euler_gravity_coriolis():
     _check_tendency(Case("euler_gravity", 3, 3, 2, form="weak", sflux="rusanov", bc="reflect",
                          periodic=(True, True, False), gravity_domain=True, source="coriolis",
